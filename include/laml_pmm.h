@@ -1,0 +1,159 @@
+/*
+ * laml_pmm.h -- C ABI of the MI355X (gfx950) PMM likelihood / EM engine.
+ *
+ * The reference (raphael-group/LAML 1.0.5) is pure Python and has no FFI; the
+ * entry points below are what its solver classes would bind for the hot path.
+ * Each declaration names the reference function it replaces.
+ *
+ * Conventions
+ *  - Every function returns LAML_OK (0) or a positive error code; the message
+ *    of the last error on the calling thread is laml_last_error().
+ *  - No C++ exception crosses this boundary.
+ *  - "device" pointers are HIP device memory of the forest's device (e.g. the
+ *    data_ptr() of a torch tensor); "host" pointers are ordinary memory.
+ *  - Work is enqueued on the given hipStream_t (passed as void*; NULL = the
+ *    null stream) and is stream-ordered.  A forest is not thread-safe; distinct
+ *    forests are independent.
+ *
+ * Model objects
+ *  - A "forest" holds one or more trees.  Trees are grouped into "problems":
+ *    all trees of a problem share nu, phi and one character matrix (the
+ *    reference's multi-tree solver, ML_solver.py:19-32); distinct problems are
+ *    independent (batched NNI candidates, serving requests).
+ *  - Nodes are numbered in height-level order over the whole forest: level h
+ *    holds the nodes whose longest path down to a leaf has h edges, so leaves
+ *    are nodes [0, num_leaves).  Every node owns the branch above it; a root
+ *    owns the stem edge from the unedited ancestor (EM_solver.py:154-158).
+ *  - Trees must be binary (polytomies resolved on the host, EM_solver.py:26-43).
+ */
+#ifndef LAML_PMM_H
+#define LAML_PMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LAML_OK 0
+#define LAML_EINVAL 1    /* malformed description or argument */
+#define LAML_EDEVICE 2   /* HIP runtime error */
+#define LAML_ENOMEM 3    /* device allocation failed */
+#define LAML_ESTATE 4    /* call out of order */
+
+/* leaf codes (int16) */
+#define LAML_CODE_UNEDITED 0  /* state 0                                    */
+                              /* 1..m: local edit index of the site         */
+#define LAML_CODE_MISSING (-1) /* '?': dropout or silencing                 */
+#define LAML_CODE_SILENCED (-2) /* explicit -1: silencing only             */
+
+/* per-problem optimisation flags */
+#define LAML_OPT_PHI 1u
+#define LAML_OPT_NU 2u
+
+typedef struct laml_forest laml_forest;
+
+typedef struct {
+  int32_t num_problems;
+  int32_t num_trees;
+  int32_t num_nodes;
+  int32_t num_leaves;
+  int32_t num_levels;          /* levels 0..num_levels-1                     */
+  const int32_t* level_off;    /* [num_levels+1]                            */
+  const int32_t* child0;       /* [num_nodes], -1 for leaves                */
+  const int32_t* child1;       /* [num_nodes], -1 for leaves                */
+  const int32_t* parent;       /* [num_nodes], -1 for roots                 */
+  const int32_t* tree_of;      /* [num_nodes]                               */
+  const uint8_t* free_branch;  /* [num_nodes] 1 = optimised by the M-step   */
+  const int32_t* tree_root;    /* [num_trees]                               */
+  const int32_t* tree_problem; /* [num_trees], non-decreasing               */
+  const int32_t* tree_nsites;  /* [num_trees] K of the tree's problem       */
+  const int32_t* site_qoff;    /* [sum_t K_t + 1] CSR offsets into prior    */
+  const double* prior;         /* [site_qoff[last]] q of local states 1..m  */
+  const int16_t* leaf_codes;   /* [sum over leaves of K] leaf rows in node order */
+  const double* problem_nobs;  /* [num_problems] # non-'?' leaf entries (R) */
+  const double* problem_phimin;/* [num_problems] phi cut-off 1/(K*ncells)   */
+  const uint32_t* problem_flags; /* [num_problems] LAML_OPT_* bits          */
+  double dmin, dmax;           /* branch-length box of the M-step           */
+} laml_forest_desc;
+
+/* Host-only check of a description (shapes, topology, codes, offsets); no
+ * device is touched.  laml_forest_create runs the same checks first. */
+int laml_forest_validate(const laml_forest_desc* desc);
+
+/* Build the device-resident forest: uploads the description, computes the
+ * az-partition labels on the device.
+ * Replaces: ML_solver.__init__ + EM_solver.__init__ data setup
+ * (ML_solver.py:19-86, EM_solver.py:19-24) and az_partition
+ * (ML_solver.py:125-154). */
+int laml_forest_create(const laml_forest_desc* desc, int device, laml_forest** out);
+
+/* Release every device buffer of the forest. */
+void laml_forest_destroy(laml_forest* f);
+
+/* Change the LAML_OPT_* flags of every problem (host array [num_problems]).
+ * Replaces the fixed_phi / fixed_nu arguments of EM_solver.optimize_one
+ * (EM_solver.py:514-523). */
+int laml_forest_set_flags(laml_forest* f, const uint32_t* flags_host);
+
+/* Per-node az-partition labels, int16 [num_nodes * K] row-major (0 = 'z',
+ * -1 = '?', j>0 = local edit j), copied to host.  Replaces reading node.alpha
+ * after ML_solver.az_partition (ML_solver.py:125-154). */
+int laml_forest_labels(laml_forest* f, int16_t* labels_host, int64_t capacity);
+
+/* Negative log-likelihood.
+ * d: device [num_nodes]; nu, phi: device [num_problems];
+ * nll: device [num_problems] (out); site_llh: device [sum_t K_t] (out, may be
+ * NULL) per-tree per-site log-likelihood.
+ * Replaces: ML_solver.negative_llh -> EM_solver.lineage_llh -> Estep_in_llh
+ * (ML_solver.py:252-254, EM_solver.py:91-141). */
+int laml_nll(laml_forest* f, const double* d, const double* nu, const double* phi,
+             double* nll, double* site_llh, void* stream);
+
+/* E-step: inside, outside and posterior expected counts.
+ * counts: device [num_nodes * 5] (out) raw sums over sites of S0..S4 of each
+ * branch; rtilde: device [num_problems] (out) sum over missing leaf entries of
+ * P(not silenced | D); nll: device [num_problems] (out).
+ * Replaces: EM_solver.Estep = Estep_in_llh + Estep_out_llh + __out_alpha_up__
+ * + Estep_posterior (EM_solver.py:91-305) and the count sums of Mstep
+ * (EM_solver.py:318-348). */
+int laml_estep(laml_forest* f, const double* d, const double* nu, const double* phi,
+               double* counts, double* rtilde, double* nll, void* stream);
+
+/* Outside pass and posterior counts only, on the inside state left resident
+ * by the preceding laml_nll / laml_estep call on this forest with the same
+ * parameters (stream-ordered).  Outputs as laml_estep.
+ * Replaces: EM_solver.Estep_out_llh + __out_alpha_up__ + Estep_posterior
+ * (EM_solver.py:143-300). */
+int laml_outside(laml_forest* f, const double* phi, double* counts, double* rtilde, void* stream);
+
+/* M-step, in place on d, nu, phi (device): phi closed form, per-branch exact
+ * maximisation of the branch-length objective over [dmin, dmax] for free
+ * branches, then the 1-D nu problem on [0, 1].  counts/rtilde as produced by
+ * laml_estep.
+ * Replaces: EM_solver.Mstep without ultrametric constraint
+ * (EM_solver.py:307-459; MOSEK calls at :365, :405). */
+int laml_mstep(laml_forest* f, const double* counts, const double* rtilde,
+               double* d, double* nu, double* phi, void* stream);
+
+/* Full EM loop on the device, stopping each problem at
+ * |dllh/llh| < tol or after max_iter M-steps (tol <= 0: run max_iter).
+ * d, nu, phi: device, updated in place.  nll: device [num_problems] final
+ * negative log-likelihood; iters: device int32 [num_problems] EM iteration
+ * count with the reference's numbering; trace: device [(max_iter+1) *
+ * num_problems] or NULL, negative log-likelihood after each M-step (row 0 =
+ * start).
+ * Replaces: EM_solver.EM_optimization (EM_solver.py:461-512). */
+int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter, double tol,
+            double* nll, int32_t* iters, double* trace, void* stream);
+
+/* Text of the last error on this thread ("" if none). */
+const char* laml_last_error(void);
+
+/* Library version string. */
+const char* laml_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LAML_PMM_H */

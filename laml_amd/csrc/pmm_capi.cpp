@@ -1,0 +1,442 @@
+// C ABI of the PMM engine: forest construction (host validation + upload) and
+// the stream-ordered entry points declared in include/laml_pmm.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "laml_pmm.h"
+#include "pmm_kernels.hpp"
+
+using laml::DevForest;
+using laml::EmState;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define LAML_HIP(expr)                                                                            \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess) return fail(LAML_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+struct laml_forest {
+  int device = 0;
+  int W = 64;
+  DevForest F{};
+  std::vector<int32_t> level_off;  // host copy
+  std::vector<int32_t> tree_nsites;
+  std::vector<void*> allocs;
+  int64_t leaf_rows = 0, int_rows = 0, total_sites = 0;
+  // device scratch owned by the forest
+  double* counts = nullptr;  // [N*5]
+  double* rtilde = nullptr;  // [P]
+  double* phi_buf = nullptr; // [P] staging when the caller passes phi
+  EmState em{};
+  int32_t* em_done = nullptr;
+
+  template <typename T>
+  int alloc(T** p, size_t n) {
+    void* q = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) return fail(LAML_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return LAML_OK;
+  }
+  template <typename T>
+  int upload(const T** dst, const T* src, size_t n) {
+    T* p = nullptr;
+    int rc = alloc(&p, n);
+    if (rc) return rc;
+    if (n) LAML_HIP(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    *dst = p;
+    return LAML_OK;
+  }
+  ~laml_forest() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+int validate(const laml_forest_desc* d, std::vector<int64_t>& row_off, int64_t& leaf_rows, int64_t& int_rows,
+             std::vector<int32_t>& prob_tree_off, std::vector<int32_t>& prob_node_off,
+             std::vector<int32_t>& prob_nodes, std::vector<int32_t>& tree_site_base, int& max_k) {
+  if (!d) return fail(LAML_EINVAL, "null description");
+  const int N = d->num_nodes, n = d->num_leaves, T = d->num_trees, P = d->num_problems, H = d->num_levels;
+  if (N <= 0 || n <= 0 || T <= 0 || P <= 0 || H <= 0 || n > N)
+    return fail(LAML_EINVAL, "non-positive sizes");
+  if (!d->level_off || !d->child0 || !d->child1 || !d->parent || !d->tree_of || !d->free_branch ||
+      !d->tree_root || !d->tree_problem || !d->tree_nsites || !d->site_qoff || !d->prior || !d->leaf_codes ||
+      !d->problem_nobs || !d->problem_phimin || !d->problem_flags)
+    return fail(LAML_EINVAL, "null array in description");
+  if (d->level_off[0] != 0 || d->level_off[H] != N || d->level_off[1] != n)
+    return fail(LAML_EINVAL, "level_off must start at 0, end at num_nodes and have level 0 = leaves");
+  std::vector<int32_t> level(N, -1);
+  for (int h = 0; h < H; ++h) {
+    if (d->level_off[h + 1] < d->level_off[h]) return fail(LAML_EINVAL, "level_off not monotone");
+    for (int v = d->level_off[h]; v < d->level_off[h + 1]; ++v) level[v] = h;
+  }
+  for (int t = 0; t < T; ++t) {
+    if (d->tree_problem[t] < 0 || d->tree_problem[t] >= P) return fail(LAML_EINVAL, "tree_problem out of range");
+    if (t > 0 && d->tree_problem[t] < d->tree_problem[t - 1]) return fail(LAML_EINVAL, "tree_problem not sorted");
+    if (d->tree_nsites[t] <= 0 || d->tree_nsites[t] > (1 << 24)) return fail(LAML_EINVAL, "bad tree_nsites");
+    const int r = d->tree_root[t];
+    if (r < 0 || r >= N || d->parent[r] != -1 || d->tree_of[r] != t) return fail(LAML_EINVAL, "bad tree_root");
+  }
+  // every problem owns >= 1 tree and all its trees have the same K
+  prob_tree_off.assign(P + 1, 0);
+  for (int t = 0; t < T; ++t) prob_tree_off[d->tree_problem[t] + 1]++;
+  for (int p = 0; p < P; ++p) {
+    if (prob_tree_off[p + 1] == 0) return fail(LAML_EINVAL, "problem without trees");
+    prob_tree_off[p + 1] += prob_tree_off[p];
+  }
+  for (int t = 1; t < T; ++t)
+    if (d->tree_problem[t] == d->tree_problem[t - 1] && d->tree_nsites[t] != d->tree_nsites[t - 1])
+      return fail(LAML_EINVAL, "trees of one problem differ in site count");
+  // topology
+  for (int v = 0; v < N; ++v) {
+    const int t = d->tree_of[v];
+    if (t < 0 || t >= T) return fail(LAML_EINVAL, "tree_of out of range");
+    const int c0 = d->child0[v], c1 = d->child1[v];
+    if (v < n) {
+      if (c0 != -1 || c1 != -1) return fail(LAML_EINVAL, "leaf with children");
+    } else {
+      if (c0 < 0 || c0 >= N || c1 < 0 || c1 >= N || c0 == c1) return fail(LAML_EINVAL, "bad children");
+      if (level[c0] >= level[v] || level[c1] >= level[v]) return fail(LAML_EINVAL, "child not below parent level");
+      if (d->parent[c0] != v || d->parent[c1] != v) return fail(LAML_EINVAL, "parent/child mismatch");
+      if (d->tree_of[c0] != t || d->tree_of[c1] != t) return fail(LAML_EINVAL, "child in another tree");
+    }
+    const int pa = d->parent[v];
+    if (pa == -1) {
+      if (d->tree_root[t] != v) return fail(LAML_EINVAL, "second root in a tree");
+    } else if (pa < n || pa >= N || (d->child0[pa] != v && d->child1[pa] != v)) {
+      return fail(LAML_EINVAL, "bad parent");
+    }
+  }
+  // rows and sites
+  tree_site_base.assign(T + 1, 0);
+  max_k = 0;
+  for (int t = 0; t < T; ++t) {
+    tree_site_base[t + 1] = tree_site_base[t] + d->tree_nsites[t];
+    max_k = std::max(max_k, (int)d->tree_nsites[t]);
+  }
+  row_off.assign(N, 0);
+  leaf_rows = 0;
+  int_rows = 0;
+  for (int v = 0; v < N; ++v) {
+    const int K = d->tree_nsites[d->tree_of[v]];
+    if (v < n) {
+      row_off[v] = leaf_rows;
+      leaf_rows += K;
+    } else {
+      row_off[v] = int_rows;
+      int_rows += K;
+    }
+  }
+  const int S = tree_site_base[T];
+  if (d->site_qoff[0] != 0) return fail(LAML_EINVAL, "site_qoff[0] != 0");
+  for (int s = 0; s < S; ++s)
+    if (d->site_qoff[s + 1] < d->site_qoff[s]) return fail(LAML_EINVAL, "site_qoff not monotone");
+  for (int v = 0; v < n; ++v) {
+    const int t = d->tree_of[v];
+    const int K = d->tree_nsites[t];
+    for (int k = 0; k < K; ++k) {
+      const int c = d->leaf_codes[row_off[v] + k];
+      const int s = tree_site_base[t] + k;
+      const int m = d->site_qoff[s + 1] - d->site_qoff[s];
+      if (c < -2 || c > m) return fail(LAML_EINVAL, "leaf code out of range for its site");
+    }
+  }
+  // nodes grouped by problem
+  prob_node_off.assign(P + 1, 0);
+  for (int v = 0; v < N; ++v) prob_node_off[d->tree_problem[d->tree_of[v]] + 1]++;
+  for (int p = 0; p < P; ++p) prob_node_off[p + 1] += prob_node_off[p];
+  prob_nodes.assign(N, 0);
+  std::vector<int32_t> fill(prob_node_off.begin(), prob_node_off.end() - 1);
+  for (int v = 0; v < N; ++v) prob_nodes[fill[d->tree_problem[d->tree_of[v]]]++] = v;
+  if (!(d->dmin > 0.0) || !(d->dmax > d->dmin)) return fail(LAML_EINVAL, "bad branch-length box");
+  return LAML_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* laml_last_error(void) { return g_err.c_str(); }
+
+const char* laml_version(void) { return "laml-amd 0.1 (gfx950, fp64)"; }
+
+int laml_forest_validate(const laml_forest_desc* d) {
+  std::vector<int64_t> row_off;
+  std::vector<int32_t> prob_tree_off, prob_node_off, prob_nodes, tree_site_base;
+  int64_t leaf_rows = 0, int_rows = 0;
+  int max_k = 0;
+  return validate(d, row_off, leaf_rows, int_rows, prob_tree_off, prob_node_off, prob_nodes, tree_site_base, max_k);
+}
+
+int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out) {
+  if (!out) return fail(LAML_EINVAL, "null output pointer");
+  *out = nullptr;
+  std::vector<int64_t> row_off;
+  std::vector<int32_t> prob_tree_off, prob_node_off, prob_nodes, tree_site_base;
+  int64_t leaf_rows = 0, int_rows = 0;
+  int max_k = 0;
+  int rc = validate(d, row_off, leaf_rows, int_rows, prob_tree_off, prob_node_off, prob_nodes, tree_site_base, max_k);
+  if (rc) return rc;
+  int ndev = 0;
+  LAML_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(LAML_EINVAL, "device out of range");
+  DeviceGuard guard(device);
+  laml_forest* f = new laml_forest();
+  f->device = device;
+  f->W = laml::seg_width(max_k);
+  f->level_off.assign(d->level_off, d->level_off + d->num_levels + 1);
+  f->tree_nsites.assign(d->tree_nsites, d->tree_nsites + d->num_trees);
+  f->leaf_rows = leaf_rows;
+  f->int_rows = int_rows;
+  const int N = d->num_nodes, T = d->num_trees, P = d->num_problems;
+  const int S = tree_site_base[T];
+  f->total_sites = S;
+  DevForest& F = f->F;
+  F.num_nodes = N;
+  F.num_leaves = d->num_leaves;
+  F.num_trees = T;
+  F.num_problems = P;
+  F.dmin = d->dmin;
+  F.dmax = d->dmax;
+  std::vector<int32_t> tree_prob(d->tree_problem, d->tree_problem + T);
+#define UP(dst, src, n) \
+  if ((rc = f->upload(&F.dst, src, (size_t)(n)))) { delete f; return rc; }
+  UP(child0, d->child0, N);
+  UP(child1, d->child1, N);
+  UP(tree_of, d->tree_of, N);
+  UP(row_off, row_off.data(), N);
+  UP(free_branch, d->free_branch, N);
+  UP(tree_root, d->tree_root, T);
+  UP(tree_prob, tree_prob.data(), T);
+  UP(tree_nsites, d->tree_nsites, T);
+  UP(tree_site_base, tree_site_base.data(), T);
+  UP(prob_tree_off, prob_tree_off.data(), P + 1);
+  UP(prob_node_off, prob_node_off.data(), P + 1);
+  UP(prob_nodes, prob_nodes.data(), N);
+  UP(prob_flags, d->problem_flags, P);
+  UP(prob_nobs, d->problem_nobs, P);
+  UP(prob_phimin, d->problem_phimin, P);
+  UP(site_qoff, d->site_qoff, S + 1);
+  UP(prior, d->prior, d->site_qoff[S]);
+  UP(codes, d->leaf_codes, leaf_rows);
+#undef UP
+#define AL(dst, type, n) \
+  if ((rc = f->alloc<type>(&dst, (size_t)(n)))) { delete f; return rc; }
+  int16_t* labels;
+  laml::Coef* coef;
+  double2* beta;
+  int32_t *bE, *aE, *LE;
+  double4* alpha;
+  int16_t* aG;
+  double *Lm, *site_llh, *rt_part, *snorm;
+  AL(labels, int16_t, int_rows);
+  AL(coef, laml::Coef, N);
+  AL(beta, double2, int_rows);
+  AL(bE, int32_t, int_rows);
+  AL(alpha, double4, int_rows);
+  AL(aE, int32_t, int_rows);
+  AL(aG, int16_t, int_rows);
+  AL(Lm, double, S);
+  AL(LE, int32_t, S);
+  AL(site_llh, double, S);
+  AL(rt_part, double, N);
+  AL(snorm, double, (size_t)N * 5);
+  AL(f->counts, double, (size_t)N * 5);
+  AL(f->rtilde, double, P);
+  AL(f->phi_buf, double, P);
+  AL(f->em.done, int32_t, P);
+  AL(f->em.iters, int32_t, P);
+  AL(f->em.pre, double, P);
+#undef AL
+  F.labels = labels;
+  F.coef = coef;
+  F.beta = beta;
+  F.bE = bE;
+  F.alpha = alpha;
+  F.aE = aE;
+  F.aG = aG;
+  F.Lm = Lm;
+  F.LE = LE;
+  F.site_llh = site_llh;
+  F.rt_part = rt_part;
+  F.snorm = snorm;
+  F.phi = nullptr;
+  if (hipMemset(rt_part, 0, sizeof(double) * N) != hipSuccess) {
+    delete f;
+    return fail(LAML_EDEVICE, "hipMemset failed");
+  }
+  laml::run_labels(F, f->level_off, f->W, nullptr);
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    delete f;
+    return fail(LAML_EDEVICE, std::string("label kernels: ") + hipGetErrorString(e));
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess) {
+    delete f;
+    return fail(LAML_EDEVICE, std::string("label kernels: ") + hipGetErrorString(e));
+  }
+  *out = f;
+  return LAML_OK;
+}
+
+void laml_forest_destroy(laml_forest* f) {
+  if (!f) return;
+  DeviceGuard guard(f->device);
+  (void)hipDeviceSynchronize();
+  delete f;
+}
+
+int laml_forest_set_flags(laml_forest* f, const uint32_t* flags_host) {
+  if (!f || !flags_host) return fail(LAML_EINVAL, "null argument");
+  DeviceGuard guard(f->device);
+  LAML_HIP(hipMemcpy(const_cast<uint32_t*>(f->F.prob_flags), flags_host, sizeof(uint32_t) * f->F.num_problems,
+                     hipMemcpyHostToDevice));
+  return LAML_OK;
+}
+
+int laml_forest_labels(laml_forest* f, int16_t* labels_host, int64_t capacity) {
+  if (!f || !labels_host) return fail(LAML_EINVAL, "null argument");
+  DeviceGuard guard(f->device);
+  const int64_t need = f->leaf_rows + f->int_rows;
+  if (capacity < need) return fail(LAML_EINVAL, "labels capacity too small");
+  std::vector<int16_t> codes(f->leaf_rows), lab(f->int_rows);
+  LAML_HIP(hipMemcpy(codes.data(), f->F.codes, sizeof(int16_t) * f->leaf_rows, hipMemcpyDeviceToHost));
+  LAML_HIP(hipMemcpy(lab.data(), f->F.labels, sizeof(int16_t) * f->int_rows, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < f->leaf_rows; ++i) {
+    const int c = codes[i];
+    labels_host[i] = (int16_t)(c > 0 ? c : (c == 0 ? 0 : -1));
+  }
+  std::memcpy(labels_host + f->leaf_rows, lab.data(), sizeof(int16_t) * f->int_rows);
+  return LAML_OK;
+}
+
+static int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(LAML_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+  return LAML_OK;
+}
+
+int laml_nll(laml_forest* f, const double* d, const double* nu, const double* phi, double* nll, double* site_llh,
+             void* stream) {
+  if (!f || !d || !nu || !phi || !nll) return fail(LAML_EINVAL, "null argument");
+  DeviceGuard guard(f->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevForest F = f->F;
+  F.phi = phi;
+  laml::run_coef(F, d, nu, nullptr, s);
+  laml::run_inside(F, f->level_off, f->W, nullptr, s);
+  laml::run_root(F, f->W, site_llh, nullptr, nullptr, s);
+  EmState none{};
+  laml::run_llh(F, nll, none, 0, 0, 0.0, s);
+  return check_launch();
+}
+
+int laml_estep(laml_forest* f, const double* d, const double* nu, const double* phi, double* counts, double* rtilde,
+               double* nll, void* stream) {
+  if (!f || !d || !nu || !phi || !counts || !rtilde || !nll) return fail(LAML_EINVAL, "null argument");
+  DeviceGuard guard(f->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevForest F = f->F;
+  F.phi = phi;
+  laml::run_coef(F, d, nu, nullptr, s);
+  laml::run_inside(F, f->level_off, f->W, nullptr, s);
+  laml::run_root(F, f->W, nullptr, counts, nullptr, s);
+  EmState none{};
+  laml::run_llh(F, nll, none, 0, 0, 0.0, s);
+  laml::run_outside(F, f->level_off, f->W, counts, nullptr, s);
+  laml::run_rtilde(F, rtilde, nullptr, s);
+  return check_launch();
+}
+
+int laml_outside(laml_forest* f, const double* phi, double* counts, double* rtilde, void* stream) {
+  if (!f || !phi || !counts || !rtilde) return fail(LAML_EINVAL, "null argument");
+  DeviceGuard guard(f->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevForest F = f->F;
+  F.phi = phi;
+  laml::run_root(F, f->W, nullptr, counts, nullptr, s);
+  laml::run_outside(F, f->level_off, f->W, counts, nullptr, s);
+  laml::run_rtilde(F, rtilde, nullptr, s);
+  return check_launch();
+}
+
+int laml_mstep(laml_forest* f, const double* counts, const double* rtilde, double* d, double* nu, double* phi,
+               void* stream) {
+  if (!f || !counts || !rtilde || !d || !nu || !phi) return fail(LAML_EINVAL, "null argument");
+  DeviceGuard guard(f->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  laml::run_mstep(f->F, counts, rtilde, d, nu, phi, nullptr, s);
+  return check_launch();
+}
+
+int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter, double tol, double* nll,
+            int32_t* iters, double* trace, void* stream) {
+  if (!f || !d || !nu || !phi || !nll || !iters) return fail(LAML_EINVAL, "null argument");
+  if (max_iter < 0) return fail(LAML_EINVAL, "max_iter < 0");
+  DeviceGuard guard(f->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevForest F = f->F;
+  F.phi = phi;
+  const int P = F.num_problems;
+  EmState em = f->em;
+  em.trace = trace;
+  LAML_HIP(hipMemsetAsync(em.done, 0, sizeof(int32_t) * P, s));
+  LAML_HIP(hipMemsetAsync(em.iters, 0, sizeof(int32_t) * P, s));
+  std::vector<int32_t> done_h(P);
+  const int chunk = 8;
+  for (int step = 0; step <= max_iter; ++step) {
+    laml::run_coef(F, d, nu, em.done, s);
+    laml::run_inside(F, f->level_off, f->W, em.done, s);
+    laml::run_root(F, f->W, nullptr, f->counts, em.done, s);
+    laml::run_llh(F, nll, em, step, max_iter, tol, s);
+    if (step < max_iter) {
+      laml::run_outside(F, f->level_off, f->W, f->counts, em.done, s);
+      laml::run_rtilde(F, f->rtilde, em.done, s);
+      laml::run_mstep(F, f->counts, f->rtilde, d, nu, phi, em.done, s);
+    }
+    int rc = check_launch();
+    if (rc) return rc;
+    if (tol > 0.0 && step % chunk == chunk - 1 && step < max_iter) {
+      LAML_HIP(hipMemcpyAsync(done_h.data(), em.done, sizeof(int32_t) * P, hipMemcpyDeviceToHost, s));
+      LAML_HIP(hipStreamSynchronize(s));
+      bool all = true;
+      for (int p = 0; p < P; ++p) all = all && done_h[p];
+      if (all) break;
+    }
+  }
+  LAML_HIP(hipMemcpyAsync(iters, em.iters, sizeof(int32_t) * P, hipMemcpyDeviceToDevice, s));
+  return check_launch();
+}
+
+}  // extern "C"

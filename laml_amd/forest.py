@@ -1,0 +1,343 @@
+"""Host-side preparation of the device forest (integer work, bit-exact).
+
+A *problem* is what one reference solver object holds: one or more trees that
+share nu, phi and a character matrix (ML_solver.py:19-86).  A *forest* packs
+any number of problems for one set of device launches.  This module
+
+  - numbers the nodes of every tree in the reference's post-order (the order
+    of the branch-length vector, EM_solver.py:45-51 / 328-348);
+  - assigns height levels (longest path down to a leaf) and sorts all nodes of
+    the forest by (height, tree, post-order index), so each level is one
+    contiguous range and leaves come first;
+  - recodes each site's edit states to 1..m (CSR-packed priors) and each leaf
+    entry to an int16 code (0 unedited, j edit, -1 '?', -2 explicit -1);
+  - builds the C-ABI description and creates the device forest.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import CODE_MISSING, CODE_SILENCED, LAML_OPT_NU, LAML_OPT_PHI
+
+DMIN = 0.005  # laml_libs/core_constants.py:6
+DMAX = 10.0   # laml_libs/core_constants.py:7
+
+
+class TreeArrays:
+    """Post-order arrays of one binary tree."""
+
+    __slots__ = ("n_nodes", "n_leaves", "c0", "c1", "parent", "height", "length", "free",
+                 "leaf_post", "codes", "nodes")
+
+    def __init__(self, c0, c1, parent, length, free, codes, nodes=None):
+        self.c0 = np.asarray(c0, dtype=np.int64)
+        self.c1 = np.asarray(c1, dtype=np.int64)
+        self.parent = np.asarray(parent, dtype=np.int64)
+        self.length = np.asarray(length, dtype=np.float64)
+        self.free = np.asarray(free, dtype=np.uint8)
+        self.n_nodes = len(self.c0)
+        leaf = self.c0 < 0
+        self.leaf_post = np.nonzero(leaf)[0]
+        self.n_leaves = len(self.leaf_post)
+        self.codes = codes  # int16 [n_leaves, K] rows in leaf post-order
+        self.nodes = nodes
+        h = np.zeros(self.n_nodes, dtype=np.int64)
+        # post-order: children precede parents
+        for v in range(self.n_nodes):
+            a = self.c0[v]
+            if a >= 0:
+                b = self.c1[v]
+                h[v] = 1 + (h[a] if h[a] > h[b] else h[b])
+        self.height = h
+
+
+def tree_arrays_from_tree(tree, codes_of_leaf, free_of_node=None):
+    """Post-order arrays of a binary ``laml_amd.tree.Tree``; ``codes_of_leaf``
+    maps a leaf Node to its int16 code row."""
+    nodes = list(tree.traverse_postorder())
+    idx = {id(v): i for i, v in enumerate(nodes)}
+    N = len(nodes)
+    c0 = np.full(N, -1, dtype=np.int64)
+    c1 = np.full(N, -1, dtype=np.int64)
+    parent = np.full(N, -1, dtype=np.int64)
+    length = np.zeros(N, dtype=np.float64)
+    free = np.ones(N, dtype=np.uint8)
+    rows = []
+    for i, v in enumerate(nodes):
+        if v.children:
+            if len(v.children) != 2:
+                raise ValueError("device forest requires binary trees (resolve polytomies first)")
+            c0[i] = idx[id(v.children[0])]
+            c1[i] = idx[id(v.children[1])]
+        else:
+            rows.append(codes_of_leaf(v))
+        if v.parent is not None:
+            parent[i] = idx[id(v.parent)]
+        length[i] = 0.0 if v.edge_length is None else float(v.edge_length)
+        if free_of_node is not None:
+            free[i] = 1 if free_of_node(v) else 0
+    if N < 3:
+        raise ValueError("device forest requires trees with at least two leaves")
+    codes = np.stack(rows).astype(np.int16) if rows else np.zeros((0, 0), np.int16)
+    return TreeArrays(c0, c1, parent, length, free, codes, nodes)
+
+
+class SiteTable:
+    """Per-site edit alphabets of one problem: state -> local index 1..m, and
+    the normalised priors in CSR form."""
+
+    def __init__(self, Q):
+        self.K = len(Q)
+        self.maps = []
+        qoff = [0]
+        prior = []
+        for k, q in enumerate(Q):
+            mass0 = q.get(0, 0.0)
+            if mass0 not in (0, 0.0):
+                raise ValueError("site %d: the prior puts mass %r on the unedited state 0; "
+                                 "the PMM model needs priors over edit states only" % (k, mass0))
+            m = {}
+            for s in q:
+                if s == 0:
+                    continue
+                m[s] = len(m) + 1
+                prior.append(float(q[s]))
+            self.maps.append(m)
+            qoff.append(len(prior))
+        self.qoff = np.asarray(qoff, dtype=np.int32)
+        self.prior = np.asarray(prior, dtype=np.float64)
+
+    def encode_row(self, row):
+        out = np.empty(len(row), dtype=np.int16)
+        for k, x in enumerate(row):
+            if x == 0:
+                out[k] = 0
+            elif x == "?":
+                out[k] = CODE_MISSING
+            elif x == -1:
+                out[k] = CODE_SILENCED
+            else:
+                try:
+                    out[k] = self.maps[k][x]
+                except KeyError:
+                    raise KeyError("site %d: observed state %r has no prior" % (k, x)) from None
+        return out
+
+
+class ProblemSpec:
+    """One problem: trees (as TreeArrays), its site table and phi statistics."""
+
+    def __init__(self, trees, sites, nobs, phimin, flags=LAML_OPT_PHI | LAML_OPT_NU):
+        self.trees = trees
+        self.sites = sites
+        self.nobs = float(nobs)
+        self.phimin = float(phimin)
+        self.flags = int(flags)
+
+
+class ForestLayout:
+    """Level-ordered host arrays of a forest (no device work)."""
+
+    def __init__(self, problems, dmin=DMIN, dmax=DMAX):
+        self.problems = problems
+        trees = [t for p in problems for t in p.trees]
+        tree_prob = np.concatenate([np.full(len(p.trees), i, dtype=np.int32) for i, p in enumerate(problems)])
+        tree_K = np.asarray([p.sites.K for p in problems for _ in p.trees], dtype=np.int32)
+        T = len(trees)
+        sizes = np.asarray([t.n_nodes for t in trees], dtype=np.int64)
+        base = np.concatenate([[0], np.cumsum(sizes)])
+        N = int(base[-1])
+        height = np.concatenate([t.height for t in trees])
+        tree_of_g = np.repeat(np.arange(T, dtype=np.int64), sizes)
+        post_idx = np.concatenate([np.arange(s, dtype=np.int64) for s in sizes])
+        order = np.lexsort((post_idx, tree_of_g, height))  # level order -> global post id
+        inv = np.empty(N, dtype=np.int64)
+        inv[order] = np.arange(N, dtype=np.int64)
+        self.order = order
+        self.inv = inv
+        self.tree_base = base
+
+        def gmap(a, t_idx):
+            return np.where(a >= 0, a + base[t_idx], -1)
+
+        c0 = np.concatenate([gmap(t.c0, i) for i, t in enumerate(trees)])
+        c1 = np.concatenate([gmap(t.c1, i) for i, t in enumerate(trees)])
+        par = np.concatenate([gmap(t.parent, i) for i, t in enumerate(trees)])
+        free = np.concatenate([t.free for t in trees])
+
+        def relabel(a):
+            a = a[order]
+            return np.where(a >= 0, inv[np.maximum(a, 0)], -1).astype(np.int32)
+
+        self.child0 = relabel(c0)
+        self.child1 = relabel(c1)
+        self.parent = relabel(par)
+        self.tree_of = tree_of_g[order].astype(np.int32)
+        self.free = free[order].astype(np.uint8)
+        hs = height[order]
+        H = int(hs.max()) + 1
+        self.num_levels = H
+        self.level_off = np.searchsorted(hs, np.arange(H + 1)).astype(np.int32)
+        self.num_leaves = int(self.level_off[1])
+        self.tree_root = np.asarray([inv[base[i] + t.n_nodes - 1] for i, t in enumerate(trees)], dtype=np.int32)
+        leaf_glob = order[: self.num_leaves]
+        leaf_tree = tree_of_g[leaf_glob]
+        leaf_post = post_idx[leaf_glob]
+        code_chunks = []
+        for i, t in enumerate(trees):
+            sel = leaf_tree == i
+            rank = np.searchsorted(t.leaf_post, leaf_post[sel])
+            code_chunks.append(np.ascontiguousarray(t.codes[rank]).reshape(-1))
+        self.codes = np.concatenate(code_chunks).astype(np.int16) if code_chunks else np.zeros(0, np.int16)
+        qoff_parts = []
+        prior_parts = []
+        off = 0
+        for p in problems:
+            for _ in p.trees:
+                qoff_parts.append(p.sites.qoff[:-1].astype(np.int64) + off)
+                prior_parts.append(p.sites.prior)
+                off += len(p.sites.prior)
+        self.site_qoff = np.concatenate(qoff_parts + [np.asarray([off], dtype=np.int64)]).astype(np.int32)
+        self.prior = np.concatenate(prior_parts) if prior_parts else np.zeros(0)
+        self.num_nodes = N
+        self.num_trees = T
+        self.num_problems = len(problems)
+        self.tree_prob = tree_prob
+        self.tree_K = tree_K
+        self.site_base = np.concatenate([[0], np.cumsum(tree_K)]).astype(np.int64)
+        self.nobs = np.asarray([p.nobs for p in problems], dtype=np.float64)
+        self.phimin = np.asarray([p.phimin for p in problems], dtype=np.float64)
+        self.flags = np.asarray([p.flags for p in problems], dtype=np.uint32)
+        self.dmin = float(dmin)
+        self.dmax = float(dmax)
+
+    def desc(self):
+        """C-ABI description pointing into this layout's arrays."""
+        def P(a, ct):
+            return a.ctypes.data_as(ctypes.POINTER(ct))
+
+        d = _lib.ForestDesc()
+        d.num_problems = self.num_problems
+        d.num_trees = self.num_trees
+        d.num_nodes = self.num_nodes
+        d.num_leaves = self.num_leaves
+        d.num_levels = self.num_levels
+        d.level_off = P(self.level_off, ctypes.c_int32)
+        d.child0 = P(self.child0, ctypes.c_int32)
+        d.child1 = P(self.child1, ctypes.c_int32)
+        d.parent = P(self.parent, ctypes.c_int32)
+        d.tree_of = P(self.tree_of, ctypes.c_int32)
+        d.free_branch = P(self.free, ctypes.c_uint8)
+        d.tree_root = P(self.tree_root, ctypes.c_int32)
+        d.tree_problem = P(self.tree_prob, ctypes.c_int32)
+        d.tree_nsites = P(self.tree_K, ctypes.c_int32)
+        d.site_qoff = P(self.site_qoff, ctypes.c_int32)
+        d.prior = P(self.prior, ctypes.c_double)
+        d.leaf_codes = P(self.codes, ctypes.c_int16)
+        d.problem_nobs = P(self.nobs, ctypes.c_double)
+        d.problem_phimin = P(self.phimin, ctypes.c_double)
+        d.problem_flags = P(self.flags, ctypes.c_uint32)
+        d.dmin = self.dmin
+        d.dmax = self.dmax
+        return d
+
+    def validate(self):
+        """Run the C library's host-side validation of the description."""
+        lib = _lib.load()
+        d = self.desc()
+        _lib.check(lib.laml_forest_validate(ctypes.byref(d)))
+
+    # ---------------- parameter layout ----------------
+    def lengths_to_level(self, per_tree_lengths):
+        """Concatenated post-order branch lengths -> level-order vector."""
+        g = np.concatenate(per_tree_lengths)
+        return g[self.order]
+
+    def level_to_tree_lengths(self, d_level):
+        g = np.empty(self.num_nodes, dtype=np.float64)
+        g[self.order] = d_level
+        return [g[self.tree_base[i]:self.tree_base[i + 1]] for i in range(self.num_trees)]
+
+
+class Forest(ForestLayout):
+    """Device-resident forest built from ProblemSpecs."""
+
+    def __init__(self, problems, device=0, dmin=DMIN, dmax=DMAX):
+        super().__init__(problems, dmin=dmin, dmax=dmax)
+        lib = _lib.load()
+        self.device = torch.device("cuda", device)
+        self.desc_ = self.desc()
+        handle = ctypes.c_void_p()
+        _lib.check(lib.laml_forest_create(ctypes.byref(self.desc_), int(device), ctypes.byref(handle)))
+        self.handle = handle
+        self._lib = lib
+        N = self.num_nodes
+        dev = self.device
+        self.counts = torch.zeros(N * 5, dtype=torch.float64, device=dev)
+        self.rtilde = torch.zeros(self.num_problems, dtype=torch.float64, device=dev)
+        self.nll_buf = torch.zeros(self.num_problems, dtype=torch.float64, device=dev)
+        self.site_llh = torch.zeros(int(self.site_base[-1]), dtype=torch.float64, device=dev)
+        self.iters_buf = torch.zeros(self.num_problems, dtype=torch.int32, device=dev)
+
+    # ------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.laml_forest_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def set_flags(self, flags):
+        a = np.asarray(flags, dtype=np.uint32)
+        _lib.check(self._lib.laml_forest_set_flags(self.handle, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+
+    def labels(self):
+        """az-partition labels: (leaf rows, internal rows) int16 flat arrays in
+        level order, row lengths K of the node's tree."""
+        K = self.tree_K[self.tree_of]
+        leaf_rows = int(K[: self.num_leaves].sum())
+        int_rows = int(K[self.num_leaves:].sum())
+        out = np.zeros(leaf_rows + int_rows, dtype=np.int16)
+        _lib.check(self._lib.laml_forest_labels(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)),
+                                                out.size))
+        return out[:leaf_rows], out[leaf_rows:]
+
+    # ---------------- calls (device tensors in, device tensors out) -------
+    def nll(self, d, nu, phi, site_llh=False):
+        out = self.nll_buf
+        sl = self.site_llh if site_llh else None
+        _lib.check(self._lib.laml_nll(self.handle, _lib.ptr(d), _lib.ptr(nu), _lib.ptr(phi), _lib.ptr(out),
+                                      _lib.ptr(sl), self.stream()))
+        return out
+
+    def estep(self, d, nu, phi):
+        _lib.check(self._lib.laml_estep(self.handle, _lib.ptr(d), _lib.ptr(nu), _lib.ptr(phi),
+                                        _lib.ptr(self.counts), _lib.ptr(self.rtilde), _lib.ptr(self.nll_buf),
+                                        self.stream()))
+        return self.counts, self.rtilde, self.nll_buf
+
+    def outside(self, phi):
+        _lib.check(self._lib.laml_outside(self.handle, _lib.ptr(phi), _lib.ptr(self.counts), _lib.ptr(self.rtilde),
+                                          self.stream()))
+        return self.counts, self.rtilde
+
+    def mstep(self, d, nu, phi, counts=None, rtilde=None):
+        c = self.counts if counts is None else counts
+        r = self.rtilde if rtilde is None else rtilde
+        _lib.check(self._lib.laml_mstep(self.handle, _lib.ptr(c), _lib.ptr(r), _lib.ptr(d), _lib.ptr(nu),
+                                        _lib.ptr(phi), self.stream()))
+
+    def em(self, d, nu, phi, max_iter, tol, trace=None):
+        _lib.check(self._lib.laml_em(self.handle, _lib.ptr(d), _lib.ptr(nu), _lib.ptr(phi), int(max_iter),
+                                     float(tol), _lib.ptr(self.nll_buf), _lib.ptr(self.iters_buf),
+                                     _lib.ptr(trace), self.stream()))
+        return self.nll_buf, self.iters_buf

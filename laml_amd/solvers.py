@@ -1,0 +1,432 @@
+"""Drop-in solver classes with the surface of LAML's solvers.
+
+``EM_solver`` mirrors laml_libs/EM_solver.py (constructor, negative_llh,
+Estep, Mstep, EM_optimization, optimize_one, optimize, score_tree,
+get_tree_newick, get_params).  Tree handling, seeding and the optimisation
+driver stay on the host exactly as in the reference; every likelihood, E-step
+and M-step evaluation runs in the HIP kernels of ``liblaml_pmm.so`` on the GPU.
+There is no CPU fallback: constructing the device forest without the library
+or without a GPU raises.
+
+``fastEM_solver`` keeps the reference's fastEM constructor/score_tree surface
+but evaluates with the same exact FP64 engine (the reference's fastEM is an
+FP32 approximation with a different M-step, laml/fast_em/laml.py:62-110).
+"""
+import random as _random
+from copy import deepcopy
+from math import isfinite
+
+import numpy as np
+import torch
+
+from ._lib import LAML_OPT_NU, LAML_OPT_PHI
+from .forest import DMAX, DMIN, Forest, ProblemSpec, SiteTable, tree_arrays_from_tree
+from .lca import find_LCAs
+from .tree import read_tree_newick
+
+# laml_libs/core_constants.py:3-10
+MIN_LLH = -float("inf")
+EPS = 1e-10
+CONV_EPS = 1e-8
+NNI_CONV_EPS = 1e-15
+CHKPT_FREQ = 10
+DEFAULT_STRATEGY = {'resolve_search_only': False, 'only_marked': False, 'ultra_constr': False,
+                    'fixed_phi': None, 'fixed_nu': None, 'local_brlen_opt': True,
+                    'nodes_to_recompute': None}
+
+
+class Params:
+    def __init__(self, nu, phi):
+        self.nu = nu
+        self.phi = phi
+
+
+def _device_index():
+    if not torch.cuda.is_available():
+        raise RuntimeError("laml_amd solvers need a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.cuda.current_device()
+
+
+class EM_solver:
+    """GPU-backed replacement of laml_libs.EM_solver.EM_solver."""
+
+    def __init__(self, treeList, data, prior, params={'nu': 0, 'phi': 0, 'sigma': 0}):
+        # ML_solver.__init__ (ML_solver.py:19-86)
+        self.solver_name = "EM_solver"
+        charMtrx = data['charMtrx']
+        Q = prior['Q']
+        self.trees = []
+        self.num_edges = 0
+        for tree in treeList:
+            t = read_tree_newick(tree) if isinstance(tree, str) else tree
+            t.suppress_unifurcations()
+            self.num_edges += sum(1 for _ in t.traverse_postorder())
+            self.trees.append(t)
+        site_matrix = list(zip(*charMtrx.values()))
+        uninformative = [i for i, site in enumerate(site_matrix) if all(v in (0, -1, '?') for v in site)]
+        if len(Q) != len(site_matrix):
+            print(f"Warning: Q has {len(Q)} entries, {len(site_matrix)} sites in charMtrx. "
+                  f"Identified {len(uninformative)} uninformative sites.")
+            if len(uninformative) == max(len(Q), len(site_matrix)) - min(len(Q), len(site_matrix)):
+                keep = [i for i in range(len(site_matrix)) if i not in set(uninformative)]
+                charMtrx = {c: [v[i] for i in keep] for c, v in charMtrx.items()}
+            else:
+                print("Warning: Removing the uninformative sites would not help us resolve the discrepancy "
+                      "between the priors and the character matrix, so let's not change anything.")
+        self.charMtrx = charMtrx
+        self.Q = []
+        for Qi in Q:
+            s = sum(Qi[x] for x in Qi)
+            self.Q.append({x: Qi[x] / s for x in Qi})
+        self.params = Params(params['nu'], params['phi'])
+        self.numsites = len(self.charMtrx[next(iter(self.charMtrx.keys()))])
+        self.dmin = DMIN
+        self.dmax = DMAX
+        # EM_solver.__init__ / __mark_polytomies__ (EM_solver.py:19-43)
+        self.has_polytomy = False
+        self.__mark_polytomies__(eps_len=self.dmin * 0.01)
+        self.num_edges = sum(sum(1 for _ in t.traverse_postorder()) for t in self.trees)
+        for t in self.trees:
+            for v in t.traverse_postorder():
+                if not hasattr(v, 'mark_fixed'):
+                    v.mark_fixed = False
+        self._forest = None
+        self._forest_key = None
+        self._sites = None
+
+    # ------------------------------------------------------------------
+    def __mark_polytomies__(self, eps_len=0):
+        self.has_polytomy = False
+        self.num_polytomy_mark = 0
+        for tree in self.trees:
+            for node in tree.traverse_preorder():
+                node.polytomy_mark = False
+                if len(node.children) > 2:
+                    self.has_polytomy = True
+            for nn in tree.resolve_polytomies():
+                nn.polytomy_mark = True
+                nn.edge_length = eps_len
+                self.num_polytomy_mark += 1
+                self.has_polytomy = True
+
+    def get_tree_newick(self):
+        return [tree.newick() for tree in self.trees]
+
+    def get_params(self):
+        return {'phi': self.params.phi, 'nu': self.params.nu}
+
+    # ---------------- device forest ----------------
+    def _site_table(self):
+        if self._sites is None:
+            self._sites = SiteTable(self.Q)
+        return self._sites
+
+    def _topology_key(self):
+        key = []
+        for t in self.trees:
+            key.append(tuple((id(v), id(v.parent), bool(v.polytomy_mark), bool(getattr(v, 'mark_fixed', False)))
+                             for v in t.traverse_postorder()))
+        return tuple(key)
+
+    def _problem_spec(self, flags):
+        sites = self._site_table()
+        rows = {}
+
+        def codes_of(v):
+            if v.label not in self.charMtrx:
+                raise KeyError("leaf %r is not in the character matrix" % (v.label,))
+            r = rows.get(v.label)
+            if r is None:
+                r = sites.encode_row(self.charMtrx[v.label])
+                rows[v.label] = r
+            return r
+
+        def free_of(v):
+            return not v.polytomy_mark and not getattr(v, 'mark_fixed', False)
+
+        arrays = [tree_arrays_from_tree(t, codes_of, free_of) for t in self.trees]
+        nobs = 0
+        for t in self.trees:
+            for v in t.traverse_leaves():
+                nobs += sum(1 for x in self.charMtrx[v.label] if x != '?')
+        phimin = 1.0 / (self.numsites * len(self.charMtrx))
+        return ProblemSpec(arrays, sites, nobs, phimin, flags)
+
+    def _forest_for(self, flags=LAML_OPT_PHI | LAML_OPT_NU):
+        key = self._topology_key()
+        if self._forest is None or self._forest_key != key:
+            dev = _device_index()
+            if self._forest is not None:
+                self._forest.close()
+            spec = self._problem_spec(flags)
+            self._forest = Forest([spec], device=dev, dmin=self.dmin, dmax=self.dmax)
+            self._forest_key = key
+            self._forest_flags = flags
+        elif self._forest_flags != flags:
+            self._forest.set_flags([flags])
+            self._forest_flags = flags
+        return self._forest
+
+    def _device_params(self, F):
+        lengths = [np.asarray([0.0 if v.edge_length is None else float(v.edge_length)
+                               for v in t.traverse_postorder()]) for t in self.trees]
+        d = torch.tensor(F.lengths_to_level(lengths), dtype=torch.float64, device=F.device)
+        nu = torch.tensor([float(self.params.nu)], dtype=torch.float64, device=F.device)
+        phi = torch.tensor([float(self.params.phi)], dtype=torch.float64, device=F.device)
+        return d, nu, phi
+
+    def _store_params(self, F, d, nu, phi, only_free=True):
+        per_tree = F.level_to_tree_lengths(d.cpu().numpy())
+        for t, L in zip(self.trees, per_tree):
+            for v, x in zip(t.traverse_postorder(), L):
+                if only_free and (v.polytomy_mark or getattr(v, 'mark_fixed', False)):
+                    continue
+                v.edge_length = float(x)
+        self.params.nu = float(nu.item())
+        self.params.phi = float(phi.item())
+
+    # ---------------- likelihood ----------------
+    def az_partition(self):
+        """Builds the device forest; az-partition labels are computed on the
+        device (ML_solver.py:125-154) and can be read with node_labels()."""
+        self._forest_for()
+
+    def node_labels(self):
+        """Per-tree dict node -> list of labels ('z', '?', or the edit state),
+        as node.alpha of the reference's az_partition."""
+        F = self._forest_for()
+        leaf_l, int_l = F.labels()
+        sites = self._site_table()
+        inv_maps = [{j: s for s, j in m.items()} for m in sites.maps]
+        K = self.numsites
+        out = []
+        lvl_of_post = F.inv
+        base = F.tree_base
+        nl = F.num_leaves
+        for ti, t in enumerate(self.trees):
+            res = {}
+            for pi, v in enumerate(t.traverse_postorder()):
+                g = lvl_of_post[base[ti] + pi]
+                row = leaf_l[g * K:(g + 1) * K] if g < nl else int_l[(g - nl) * K:(g - nl + 1) * K]
+                res[v] = ['z' if x == 0 else ('?' if x < 0 else inv_maps[k][int(x)]) for k, x in enumerate(row)]
+            out.append(res)
+        return out
+
+    def lineage_llh(self):
+        F = self._forest_for()
+        d, nu, phi = self._device_params(F)
+        return -float(F.nll(d, nu, phi).item())
+
+    def __llh__(self):
+        return self.lineage_llh()
+
+    def negative_llh(self):
+        """ML_solver.negative_llh (ML_solver.py:252-254)."""
+        return -self.lineage_llh()
+
+    def site_llh(self):
+        """Per-site log-likelihood of each tree, list of arrays [K]."""
+        F = self._forest_for()
+        d, nu, phi = self._device_params(F)
+        F.nll(d, nu, phi, site_llh=True)
+        a = F.site_llh.cpu().numpy()
+        return [a[F.site_base[i]:F.site_base[i + 1]] for i in range(F.num_trees)]
+
+    # ---------------- EM ----------------
+    def Estep(self):
+        """EM_solver.Estep (EM_solver.py:302-305) on the device.  Stores the
+        per-branch sums over sites of S0..S4 in node.S_sum and returns
+        (nllh, R_tilde)."""
+        F = self._forest_for()
+        d, nu, phi = self._device_params(F)
+        counts, rt, nll = F.estep(d, nu, phi)
+        c = counts.view(-1, 5).cpu().numpy()
+        base = F.tree_base
+        for ti, t in enumerate(self.trees):
+            for pi, v in enumerate(t.traverse_postorder()):
+                v.S_sum = c[F.inv[base[ti] + pi]].tolist()
+        self._last_estep = (d, nu, phi)
+        return float(nll.item()), float(rt.item())
+
+    def Mstep(self, optimize_phi=True, optimize_nu=True, verbose=1, eps_nu=1e-5, eps_s=1e-6,
+              ultra_constr_cache=None, local_brlen_opt=True):
+        """EM_solver.Mstep (EM_solver.py:307-459) on the device, after Estep."""
+        if ultra_constr_cache is not None:
+            raise NotImplementedError("ultrametric-constrained M-step is not available on the device path")
+        flags = (LAML_OPT_PHI if optimize_phi else 0) | (LAML_OPT_NU if optimize_nu else 0)
+        F = self._forest_for(flags)
+        d, nu, phi = self._device_params(F)
+        F.mstep(d, nu, phi)
+        self._store_params(F, d, nu, phi)
+        return True, "optimal"
+
+    def EM_optimization(self, verbose=1, optimize_phi=True, optimize_nu=True, ultra_constr=False, maxIter=1000):
+        """EM_solver.EM_optimization (EM_solver.py:461-512), whole loop on
+        the device.  Returns (nllh, em_iter, status)."""
+        if ultra_constr:
+            raise NotImplementedError("ultrametric-constrained EM is not available on the device path")
+        flags = (LAML_OPT_PHI if optimize_phi else 0) | (LAML_OPT_NU if optimize_nu else 0)
+        F = self._forest_for(flags)
+        d, nu, phi = self._device_params(F)
+        if verbose >= 0:
+            pre = F.nll(d, nu, phi).item()
+            print("Initial phi: " + str(self.params.phi) + ". Initial nu: " + str(self.params.nu)
+                  + ". Initial nllh: " + str(pre))
+        nll, iters = F.em(d, nu, phi, maxIter, CONV_EPS)
+        self._store_params(F, d, nu, phi)
+        nllh = float(nll.item())
+        em_iter = int(iters.item())
+        if em_iter > maxIter and verbose >= 0:
+            print("Warning: exceeded maximum number of EM iterations (" + str(maxIter) + " iters)!")
+        return nllh, em_iter, "optimal"
+
+    # ---------------- initial points (ML_solver.py:195-247) ----------------
+    def ini_brlens(self):
+        x = [_random.random() * (self.dmax / 2 - 2 * self.dmin) + 2 * self.dmin for _ in range(self.num_edges)]
+        idx = 0
+        for tree in self.trees:
+            for node in tree.traverse_postorder():
+                if node.edge_length is not None:
+                    x[idx] = max(2 * self.dmin, node.edge_length)
+                idx += 1
+        return x
+
+    def ini_nu(self, fixed_nu=None):
+        return _random.random() * 0.99 if fixed_nu is None else fixed_nu
+
+    def ini_phi(self, fixed_phi=None):
+        return _random.random() * 0.99 if fixed_phi is None else fixed_phi
+
+    def ini_all(self, fixed_phi=None, fixed_nu=None):
+        return self.ini_brlens() + [self.ini_nu(fixed_nu=fixed_nu), self.ini_phi(fixed_phi=fixed_phi)]
+
+    def x2brlen(self, x):
+        i = 0
+        for tree in self.trees:
+            for node in tree.traverse_postorder():
+                if not node.polytomy_mark:
+                    node.edge_length = x[i]
+                    i += 1
+
+    def x2nu(self, x, fixed_nu=None):
+        self.params.nu = x[self.num_edges] if fixed_nu is None else fixed_nu
+
+    def x2phi(self, x, fixed_phi=None):
+        self.params.phi = x[self.num_edges + 1] if fixed_phi is None else fixed_phi
+
+    def x2params(self, x, fixed_nu=None, fixed_phi=None):
+        self.x2brlen(x)
+        self.x2nu(x, fixed_nu=fixed_nu)
+        self.x2phi(x, fixed_phi=fixed_phi)
+
+    # ---------------- drivers ----------------
+    def optimize_one(self, randseed, fixed_phi=None, fixed_nu=None, verbose=1, ultra_constr=False):
+        """EM_solver.optimize_one (EM_solver.py:514-527)."""
+        _random.seed(a=randseed)
+        x0 = self.ini_all(fixed_phi=fixed_phi, fixed_nu=fixed_nu)
+        self.x2params(x0, fixed_phi=fixed_phi, fixed_nu=fixed_nu)
+        self.az_partition()
+        nllh, em_iter, status = self.EM_optimization(verbose=verbose, optimize_phi=(fixed_phi is None),
+                                                     optimize_nu=(fixed_nu is None), ultra_constr=ultra_constr)
+        if verbose >= 0:
+            print("EM finished after " + str(em_iter) + " iterations.")
+            print("Optimal phi: " + str(self.params.phi) + ". Optimal nu: " + str(self.params.nu)
+                  + ". Optimal nllh: " + str(nllh))
+        self.em_iterations = em_iter
+        return nllh, status
+
+    def optimize(self, initials=20, fixed_phi=None, fixed_nu=None, fixed_brlen=None, verbose=1, max_trials=100,
+                 random_seeds=None, ultra_constr=False):
+        """ML_solver.optimize (ML_solver.py:256-336)."""
+        results = []
+        all_failed = True
+        all_trials = 0
+        if random_seeds is None:
+            rseeds = [int(_random.random() * 10000) for _ in range(initials)]
+        elif type(random_seeds) == int:
+            if verbose >= 0:
+                print("Global random seed: " + str(random_seeds))
+            _random.seed(a=random_seeds)
+            rseeds = [int(_random.random() * 10000) for _ in range(initials)]
+        elif type(random_seeds) == list:
+            if len(random_seeds) < initials:
+                if verbose >= 0:
+                    print("Fatal: the number of random seeds is smaller than the number of initials!")
+                return None
+            rseeds = random_seeds[:initials]
+        else:
+            if verbose >= 0:
+                print("Fatal: incorrect random_seeds type provided")
+            return None
+        while all_failed and all_trials < max_trials:
+            for rep in range(initials):
+                randseed = rseeds[rep] + all_trials
+                if verbose >= 0:
+                    print("Initial point " + str(rep + 1) + ". Random seed: " + str(randseed))
+                for t, tree in enumerate(self.trees):
+                    for node in tree.traverse_postorder():
+                        node.mark_fixed = False
+                    if fixed_brlen is None:
+                        continue
+                    fixed_nodes = find_LCAs(tree, list(fixed_brlen[t].keys()))
+                    for i, (a, b) in enumerate(fixed_brlen[t]):
+                        u = fixed_nodes[i]
+                        u.edge_length = fixed_brlen[t][(a, b)]
+                        u.mark_fixed = True
+                nllh, status = self.optimize_one(randseed, fixed_phi=fixed_phi, fixed_nu=fixed_nu, verbose=verbose,
+                                                 ultra_constr=ultra_constr)
+                if nllh is not None and isfinite(nllh):
+                    all_failed = False
+                    processed = []
+                    for tree in self.trees:
+                        tc = read_tree_newick(tree.newick())
+                        tc.collapse_short_branches(self.dmin * 0.01)
+                        processed.append(tc.newick())
+                    results.append((nllh, rep, deepcopy(self.params), processed, status))
+                elif verbose >= 0:
+                    print("Fatal: failed to optimize using initial point " + str(rep + 1))
+            all_trials += initials
+        if all_failed:
+            if verbose >= 0:
+                print("Fatal: Optimization failed on more than " + str(max_trials) + " retries. Aborting ...")
+            return None
+        results.sort(key=lambda r: (r[0], r[1]))
+        best_nllh, _, best_params, best_trees, status = results[0]
+        self.trees = [read_tree_newick(t) for t in best_trees]
+        for t in self.trees:
+            for v in t.traverse_postorder():
+                v.polytomy_mark = False
+                v.mark_fixed = False
+        self.params = best_params
+        self._forest = None
+        if verbose >= 0:
+            print("Numerical optimization finished successfully")
+        return best_nllh, status
+
+    def score_tree(self, strategy={'ultra_constr': False, 'fixed_phi': None, 'fixed_nu': None, 'fixed_brlen': None}):
+        """ML_solver.score_tree (ML_solver.py:114-123)."""
+        nllh, status = self.optimize(initials=1, verbose=-1, ultra_constr=strategy['ultra_constr'],
+                                     fixed_phi=strategy['fixed_phi'], fixed_nu=strategy['fixed_nu'],
+                                     fixed_brlen=strategy['fixed_brlen'])
+        score = None if nllh is None else -nllh
+        return score, status
+
+    # pickling ships the host state only (no device handles), as solvers
+    # cross process boundaries in the reference's parallel search
+    def __getstate__(self):
+        st = dict(self.__dict__)
+        st['_forest'] = None
+        st['_forest_key'] = None
+        st.pop('_last_estep', None)
+        return st
+
+
+class fastEM_solver(EM_solver):
+    """Surface of laml_libs.fastEM_solver.fastEM_solver (fastEM_solver.py:149-178),
+    evaluated by the exact FP64 device engine."""
+
+    def __init__(self, treeList, data, prior, params={'nu': 0, 'phi': 0, 'sigma': 0}):
+        super().__init__(treeList, data, prior, params={'nu': 0, 'phi': 0, 'sigma': 0})
+        self.solver_name = "fastEM_solver"
+        self.data = data
+        self.prior = prior

@@ -1,0 +1,164 @@
+"""CPU tests of the host side: tree handling, forest layout (integer work,
+bit-exact), the C ABI's exported symbols and its host-side validation."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import load_test4
+from laml_amd import _lib, sim
+from laml_amd.forest import ForestLayout, SiteTable, tree_arrays_from_tree
+from laml_amd.lca import find_LCAs
+from laml_amd.tree import read_tree_newick
+from oracle.em_oracle import OracleEM, parse_newick, postorder
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    header = open(os.path.join(ROOT, "include", "laml_pmm.h")).read()
+    names = set(re.findall(r"^(?:int|void|const char\*)\s+(laml_\w+)\(", header, re.M))
+    assert names, "no declarations parsed"
+    assert names == set(_lib.SIGNATURES), names ^ set(_lib.SIGNATURES)
+    for n in names:
+        assert hasattr(lib, n)
+    assert lib.laml_version().startswith(b"laml-amd")
+
+
+def test_newick_roundtrip_and_order():
+    s = "((a:1,b:0.5)x:0.25,(c:2,d:1e-05)y:1)r:0.1;"
+    t = read_tree_newick(s)
+    assert [v.label for v in t.traverse_postorder()] == ["a", "b", "x", "c", "d", "y", "r"]
+    assert t.newick() == "((a:1,b:0.5)x:0.25,(c:2,d:1e-05)y:1)r:0.1;"
+    assert [v.label for v in t.traverse_leaves()] == ["a", "b", "c", "d"]
+
+
+def test_tree_matches_oracle_parser():
+    T, msa, Q = load_test4()
+    t = read_tree_newick(T)
+    t.suppress_unifurcations()
+    from oracle.em_oracle import suppress_unifurcations
+    o = suppress_unifurcations(parse_newick(T))
+    a = [(v.label, v.edge_length) for v in t.traverse_postorder()]
+    b = [(v.label, v.edge_length) for v in postorder(o)]
+    assert a == b
+
+
+def test_suppress_unifurcations():
+    t = read_tree_newick("(((a:1,b:1)x:1)u:2,c:1)r;")
+    t.suppress_unifurcations()
+    assert t.newick() == "(c:1,(a:1,b:1)x:3)r;"
+    t = read_tree_newick("((a:1,b:1)x:1)r:2;")
+    t.suppress_unifurcations()
+    assert t.newick() == "(a:1,b:1)x:3;"
+
+
+def test_resolve_polytomies_binary_and_zero_length():
+    t = read_tree_newick("(a:1,b:1,c:1,d:1)r;")
+    new = t.resolve_polytomies()
+    assert len(new) == 2
+    assert all(len(v.children) in (0, 2) for v in t.traverse_postorder())
+    assert all(v.edge_length == 0.0 for v in new)
+    assert sorted(v.label for v in t.traverse_leaves()) == ["a", "b", "c", "d"]
+
+
+def test_collapse_short_branches():
+    t = read_tree_newick("((a:1,b:1)x:0.00001,c:1)r;")
+    t.collapse_short_branches(0.00005)
+    assert t.newick() == "(a:1.00001,b:1.00001,c:1)r:0;"
+
+
+def test_find_lcas():
+    t = read_tree_newick("((a,b)x,((c,d)y,e)z)r;")
+    res = find_LCAs(t, [("a", "b"), ("c", "e"), ("a", "e"), ("d", "d"), ("q", "c")])
+    assert [v.label for v in res] == ["x", "z", "r", "d", "c"]
+
+
+def _layout_for(T, msa, Q):
+    t = read_tree_newick(T)
+    t.suppress_unifurcations()
+    sites = SiteTable(Q)
+    ta = tree_arrays_from_tree(t, lambda v: sites.encode_row(msa[v.label]))
+    from laml_amd.forest import ProblemSpec
+
+    return ForestLayout([ProblemSpec([ta], sites, 1, 0.01)]), t, sites
+
+
+def test_layout_levels_and_validation():
+    T, msa, Q = load_test4()
+    lay, t, sites = _layout_for(T, msa, Q)
+    lay.validate()
+    N = lay.num_nodes
+    # children strictly below, leaves first, parent pointers consistent
+    lvl = np.zeros(N, dtype=int)
+    for h in range(lay.num_levels):
+        lvl[lay.level_off[h]:lay.level_off[h + 1]] = h
+    for v in range(lay.num_leaves, N):
+        for c in (lay.child0[v], lay.child1[v]):
+            assert lvl[c] < lvl[v] and lay.parent[c] == v
+            assert lvl[v] == 1 + max(lvl[lay.child0[v]], lvl[lay.child1[v]])
+    assert (lay.child0[: lay.num_leaves] == -1).all()
+    # leaf codes in level order decode back to the character matrix
+    K = sites.K
+    inv_maps = [{j: s for s, j in m.items()} for m in sites.maps]
+    nodes = list(t.traverse_postorder())
+    for i, g in enumerate(lay.order[: lay.num_leaves]):
+        v = nodes[g]
+        row = lay.codes[i * K:(i + 1) * K]
+        dec = [0 if c == 0 else ("?" if c == -1 else (-1 if c == -2 else inv_maps[k][int(c)])) for k, c in
+               enumerate(row)]
+        assert dec == msa[v.label]
+
+
+def test_validation_rejects_bad_description():
+    T, msa, Q = load_test4()
+    lay, _, _ = _layout_for(T, msa, Q)
+    lay.child0[-1] = lay.child1[-1]
+    with pytest.raises(_lib.LamlError):
+        lay.validate()
+    lay, _, _ = _layout_for(T, msa, Q)
+    lay.codes[3] = 999
+    with pytest.raises(_lib.LamlError, match="leaf code"):
+        lay.validate()
+
+
+def test_prior_with_mass_on_unedited_state_rejected():
+    with pytest.raises(ValueError):
+        SiteTable([{0: 0.2, 1: 0.8}])
+
+
+def test_unknown_observed_state_rejected():
+    sites = SiteTable([{1: 1.0}])
+    with pytest.raises(KeyError):
+        sites.encode_row([7])
+
+
+def test_synthetic_generator_semantics():
+    # proportions of unedited / missing leaf entries follow the model
+    tree, Q, codes, truth = sim.make_problem(2000, 60, 5, 20, 1.0, 0.1, 0.2, 5)
+    assert codes.shape == (2000, 60)
+    depth_len = 1.0  # stem + root-to-tip
+    p0 = np.exp(-(1 + 0.1) * depth_len)  # P(still unedited at a leaf, no dropout)
+    frac0 = (codes == 0).mean()
+    assert abs(frac0 - p0 * 0.8) < 0.02
+    p_sil = 1 - np.exp(-0.1 * depth_len)
+    frac_miss = (codes == -1).mean()
+    assert abs(frac_miss - (p_sil + (1 - p_sil) * 0.2)) < 0.02
+    # same seed, same data
+    _, _, codes2, _ = sim.make_problem(2000, 60, 5, 20, 1.0, 0.1, 0.2, 5)
+    assert (codes == codes2).all()
+
+
+def test_yule_tree_is_ultrametric():
+    rng = np.random.Generator(np.random.PCG64(3))
+    t = sim.yule_tree(500, rng)
+    tr = t.to_tree()
+    dist = {}
+    for v in tr.traverse_preorder():
+        dist[v] = (v.edge_length) + (dist[v.parent] if v.parent is not None else 0.0)
+    leaves = [dist[v] for v in tr.traverse_leaves()]
+    assert max(leaves) - min(leaves) < 1e-12
+    assert abs(max(leaves) - 1.0) < 1e-12
