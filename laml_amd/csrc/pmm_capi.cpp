@@ -82,9 +82,23 @@ struct DeviceGuard {
   }
 };
 
-int validate(const laml_forest_desc* d, std::vector<int64_t>& row_off, int64_t& leaf_rows, int64_t& int_rows,
-             std::vector<int32_t>& prob_tree_off, std::vector<int32_t>& prob_node_off,
-             std::vector<int32_t>& prob_nodes, std::vector<int32_t>& tree_site_base, int& max_k) {
+struct Derived {
+  std::vector<int64_t> row_off;
+  int64_t leaf_rows = 0, int_rows = 0;
+  std::vector<int32_t> prob_tree_off, prob_node_off, prob_nodes, prob_int_off, prob_nfree, node_pos, tree_site_base;
+  std::vector<int32_t> prob_blk_off;
+  int max_k = 0;
+};
+
+int validate(const laml_forest_desc* d, Derived& D) {
+  std::vector<int64_t>& row_off = D.row_off;
+  int64_t& leaf_rows = D.leaf_rows;
+  int64_t& int_rows = D.int_rows;
+  std::vector<int32_t>& prob_tree_off = D.prob_tree_off;
+  std::vector<int32_t>& prob_node_off = D.prob_node_off;
+  std::vector<int32_t>& prob_nodes = D.prob_nodes;
+  std::vector<int32_t>& tree_site_base = D.tree_site_base;
+  int& max_k = D.max_k;
   if (!d) return fail(LAML_EINVAL, "null description");
   const int N = d->num_nodes, n = d->num_leaves, T = d->num_trees, P = d->num_problems, H = d->num_levels;
   if (N <= 0 || n <= 0 || T <= 0 || P <= 0 || H <= 0 || n > N)
@@ -178,6 +192,23 @@ int validate(const laml_forest_desc* d, std::vector<int64_t>& row_off, int64_t& 
   prob_nodes.assign(N, 0);
   std::vector<int32_t> fill(prob_node_off.begin(), prob_node_off.end() - 1);
   for (int v = 0; v < N; ++v) prob_nodes[fill[d->tree_problem[d->tree_of[v]]]++] = v;
+  D.node_pos.assign(N, 0);
+  for (int i = 0; i < N; ++i) D.node_pos[prob_nodes[i]] = i;
+  D.prob_int_off.assign(P, 0);
+  D.prob_nfree.assign(P, 0);
+  for (int p = 0; p < P; ++p) {
+    int i = prob_node_off[p];
+    while (i < prob_node_off[p + 1] && prob_nodes[i] < n) ++i;
+    D.prob_int_off[p] = i;
+    for (int j = prob_node_off[p]; j < prob_node_off[p + 1]; ++j) D.prob_nfree[p] += d->free_branch[prob_nodes[j]] ? 1 : 0;
+  }
+  // nu-solve blocks: about 2048 branches per block, at most 64 per problem
+  D.prob_blk_off.assign(P + 1, 0);
+  for (int p = 0; p < P; ++p) {
+    const int nb = prob_node_off[p + 1] - prob_node_off[p];
+    const int blocks = std::max(1, std::min(64, (nb + 2047) / 2048));
+    D.prob_blk_off[p + 1] = D.prob_blk_off[p] + blocks;
+  }
   if (!(d->dmin > 0.0) || !(d->dmax > d->dmin)) return fail(LAML_EINVAL, "bad branch-length box");
   return LAML_OK;
 }
@@ -191,22 +222,23 @@ const char* laml_last_error(void) { return g_err.c_str(); }
 const char* laml_version(void) { return "laml-amd 0.1 (gfx950, fp64)"; }
 
 int laml_forest_validate(const laml_forest_desc* d) {
-  std::vector<int64_t> row_off;
-  std::vector<int32_t> prob_tree_off, prob_node_off, prob_nodes, tree_site_base;
-  int64_t leaf_rows = 0, int_rows = 0;
-  int max_k = 0;
-  return validate(d, row_off, leaf_rows, int_rows, prob_tree_off, prob_node_off, prob_nodes, tree_site_base, max_k);
+  Derived D;
+  return validate(d, D);
 }
 
 int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out) {
   if (!out) return fail(LAML_EINVAL, "null output pointer");
   *out = nullptr;
-  std::vector<int64_t> row_off;
-  std::vector<int32_t> prob_tree_off, prob_node_off, prob_nodes, tree_site_base;
-  int64_t leaf_rows = 0, int_rows = 0;
-  int max_k = 0;
-  int rc = validate(d, row_off, leaf_rows, int_rows, prob_tree_off, prob_node_off, prob_nodes, tree_site_base, max_k);
+  Derived D;
+  int rc = validate(d, D);
   if (rc) return rc;
+  std::vector<int64_t>& row_off = D.row_off;
+  std::vector<int32_t>& prob_tree_off = D.prob_tree_off;
+  std::vector<int32_t>& prob_node_off = D.prob_node_off;
+  std::vector<int32_t>& prob_nodes = D.prob_nodes;
+  std::vector<int32_t>& tree_site_base = D.tree_site_base;
+  const int64_t leaf_rows = D.leaf_rows, int_rows = D.int_rows;
+  const int max_k = D.max_k;
   int ndev = 0;
   LAML_HIP(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(LAML_EINVAL, "device out of range");
@@ -228,6 +260,8 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   F.num_problems = P;
   F.dmin = d->dmin;
   F.dmax = d->dmax;
+  F.chunk_w = f->W;
+  F.max_chunks = (max_k + f->W - 1) / f->W;
   std::vector<int32_t> tree_prob(d->tree_problem, d->tree_problem + T);
 #define UP(dst, src, n) \
   if ((rc = f->upload(&F.dst, src, (size_t)(n)))) { delete f; return rc; }
@@ -243,6 +277,11 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   UP(prob_tree_off, prob_tree_off.data(), P + 1);
   UP(prob_node_off, prob_node_off.data(), P + 1);
   UP(prob_nodes, prob_nodes.data(), N);
+  UP(prob_int_off, D.prob_int_off.data(), P);
+  UP(prob_nfree, D.prob_nfree.data(), P);
+  UP(node_pos, D.node_pos.data(), N);
+  UP(prob_blk_off, D.prob_blk_off.data(), P + 1);
+  F.num_nu_blocks = D.prob_blk_off[P];
   UP(prob_flags, d->problem_flags, P);
   UP(prob_nobs, d->problem_nobs, P);
   UP(prob_phimin, d->problem_phimin, P);
@@ -258,7 +297,8 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   int32_t *bE, *aE, *LE;
   double4* alpha;
   int16_t* aG;
-  double *Lm, *site_llh, *rt_part, *snorm;
+  double *Lm, *site_llh, *rt_part, *cpart, *nu_a, *nu_b, *nu_d, *nu_part;
+  laml::NuState* nu_state;
   AL(labels, int16_t, int_rows);
   AL(coef, laml::Coef, N);
   AL(beta, double2, int_rows);
@@ -269,8 +309,13 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   AL(Lm, double, S);
   AL(LE, int32_t, S);
   AL(site_llh, double, S);
-  AL(rt_part, double, N);
-  AL(snorm, double, (size_t)N * 5);
+  AL(rt_part, double, (size_t)N * F.max_chunks);
+  AL(cpart, double, (size_t)N * F.max_chunks * 5);
+  AL(nu_a, double, N);
+  AL(nu_b, double, N);
+  AL(nu_d, double, N);
+  AL(nu_part, double, (size_t)F.num_nu_blocks * 3);
+  AL(nu_state, laml::NuState, P);
   AL(f->counts, double, (size_t)N * 5);
   AL(f->rtilde, double, P);
   AL(f->phi_buf, double, P);
@@ -289,9 +334,16 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   F.LE = LE;
   F.site_llh = site_llh;
   F.rt_part = rt_part;
-  F.snorm = snorm;
+  F.cpart = cpart;
+  F.nu_a = nu_a;
+  F.nu_b = nu_b;
+  F.nu_d = nu_d;
+  F.nu_part = nu_part;
+  F.nu_state = nu_state;
   F.phi = nullptr;
-  if (hipMemset(rt_part, 0, sizeof(double) * N) != hipSuccess) {
+  if (hipMemset(rt_part, 0, sizeof(double) * N * F.max_chunks) != hipSuccess ||
+      hipMemset(cpart, 0, sizeof(double) * N * F.max_chunks * 5) != hipSuccess ||
+      hipMemset(nu_state, 0, sizeof(laml::NuState) * P) != hipSuccess) {
     delete f;
     return fail(LAML_EDEVICE, "hipMemset failed");
   }
@@ -356,7 +408,7 @@ int laml_nll(laml_forest* f, const double* d, const double* nu, const double* ph
   F.phi = phi;
   laml::run_coef(F, d, nu, nullptr, s);
   laml::run_inside(F, f->level_off, f->W, nullptr, s);
-  laml::run_root(F, f->W, site_llh, nullptr, nullptr, s);
+  laml::run_root(F, f->W, site_llh, nullptr, s);
   EmState none{};
   laml::run_llh(F, nll, none, 0, 0, 0.0, s);
   return check_launch();
@@ -371,10 +423,11 @@ int laml_estep(laml_forest* f, const double* d, const double* nu, const double* 
   F.phi = phi;
   laml::run_coef(F, d, nu, nullptr, s);
   laml::run_inside(F, f->level_off, f->W, nullptr, s);
-  laml::run_root(F, f->W, nullptr, counts, nullptr, s);
+  laml::run_root(F, f->W, nullptr, nullptr, s);
   EmState none{};
   laml::run_llh(F, nll, none, 0, 0, 0.0, s);
-  laml::run_outside(F, f->level_off, f->W, counts, nullptr, s);
+  laml::run_outside(F, f->level_off, f->W, nullptr, s);
+  laml::run_counts(F, counts, s);
   laml::run_rtilde(F, rtilde, nullptr, s);
   return check_launch();
 }
@@ -385,8 +438,9 @@ int laml_outside(laml_forest* f, const double* phi, double* counts, double* rtil
   hipStream_t s = static_cast<hipStream_t>(stream);
   DevForest F = f->F;
   F.phi = phi;
-  laml::run_root(F, f->W, nullptr, counts, nullptr, s);
-  laml::run_outside(F, f->level_off, f->W, counts, nullptr, s);
+  laml::run_root(F, f->W, nullptr, nullptr, s);
+  laml::run_outside(F, f->level_off, f->W, nullptr, s);
+  laml::run_counts(F, counts, s);
   laml::run_rtilde(F, rtilde, nullptr, s);
   return check_launch();
 }
@@ -418,12 +472,11 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
   for (int step = 0; step <= max_iter; ++step) {
     laml::run_coef(F, d, nu, em.done, s);
     laml::run_inside(F, f->level_off, f->W, em.done, s);
-    laml::run_root(F, f->W, nullptr, f->counts, em.done, s);
+    laml::run_root(F, f->W, nullptr, em.done, s);
     laml::run_llh(F, nll, em, step, max_iter, tol, s);
     if (step < max_iter) {
-      laml::run_outside(F, f->level_off, f->W, f->counts, em.done, s);
-      laml::run_rtilde(F, f->rtilde, em.done, s);
-      laml::run_mstep(F, f->counts, f->rtilde, d, nu, phi, em.done, s);
+      laml::run_outside(F, f->level_off, f->W, em.done, s);
+      laml::run_mstep(F, nullptr, nullptr, d, nu, phi, em.done, s);
     }
     int rc = check_launch();
     if (rc) return rc;

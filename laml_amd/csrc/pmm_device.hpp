@@ -42,6 +42,8 @@ __device__ __forceinline__ double sd_div(SD a, SD b) {
   if (a.m == 0.0) return 0.0;
   return ldexp(a.m / b.m, a.e - b.e);
 }
+// the value as a plain double (used for O(1) posteriors)
+__device__ __forceinline__ double sd_val(SD a) { return ldexp(a.m, a.e); }
 __device__ __forceinline__ double sd_log(SD a) {
   return log(a.m) + (double)a.e * 0.69314718055994530942;
 }

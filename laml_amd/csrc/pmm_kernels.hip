@@ -31,22 +31,33 @@ __global__ void k_coef(DevForest F, const double* __restrict__ d, const double* 
   F.coef[v] = c;
 }
 
+// Segment mapping shared by the level kernels: a block of 256 threads holds
+// 256/W segments of W lanes; segment s of a level handles node beg+s, and
+// blockIdx.y selects the chunk of W consecutive sites of that node.
+template <int W>
+struct Seg {
+  int v, lane, k;
+  __device__ __forceinline__ bool init(int beg, int cnt) {
+    const int seg = blockIdx.x * (256 / W) + (int)(threadIdx.x / W);
+    lane = threadIdx.x & (W - 1);
+    if (seg >= cnt) return false;
+    v = beg + seg;
+    k = (int)blockIdx.y * W + lane;
+    return true;
+  }
+};
+
 template <int W>
 __global__ void __launch_bounds__(256) k_labels(DevForest F, int beg, int cnt) {
-  const int seg = blockIdx.x * (256 / W) + (int)(threadIdx.x / W);
-  const int lane = threadIdx.x & (W - 1);
-  if (seg >= cnt) return;
-  const int v = beg + seg;
+  Seg<W> g;
+  if (!g.init(beg, cnt)) return;
+  const int v = g.v, k = g.k;
   const int t = F.tree_of[v];
-  const int K = F.tree_nsites[t];
+  if (k >= F.tree_nsites[t]) return;
   const int c1 = F.child0[v], c2 = F.child1[v];
-  const bool l1 = c1 < F.num_leaves, l2 = c2 < F.num_leaves;
-  const int64_t rv = F.row_off[v], r1 = F.row_off[c1], r2 = F.row_off[c2];
-  for (int k = lane; k < K; k += W) {
-    const int la = l1 ? code_label(F.codes[r1 + k]) : (int)F.labels[r1 + k];
-    const int lb = l2 ? code_label(F.codes[r2 + k]) : (int)F.labels[r2 + k];
-    F.labels[rv + k] = (int16_t)combine_label(la, lb);
-  }
+  const int la = c1 < F.num_leaves ? code_label(F.codes[F.row_off[c1] + k]) : (int)F.labels[F.row_off[c1] + k];
+  const int lb = c2 < F.num_leaves ? code_label(F.codes[F.row_off[c2] + k]) : (int)F.labels[F.row_off[c2] + k];
+  F.labels[F.row_off[v] + k] = (int16_t)combine_label(la, lb);
 }
 
 // load the inside state of child c at site k
@@ -68,60 +79,59 @@ __device__ __forceinline__ void load_child(const DevForest& F, bool leaf, int64_
 
 template <int W>
 __global__ void __launch_bounds__(256) k_inside(DevForest F, int beg, int cnt, const int32_t* __restrict__ done) {
-  const int seg = blockIdx.x * (256 / W) + (int)(threadIdx.x / W);
-  const int lane = threadIdx.x & (W - 1);
-  if (seg >= cnt) return;
-  const int v = beg + seg;
+  Seg<W> g;
+  if (!g.init(beg, cnt)) return;
+  const int v = g.v, k = g.k;
   const int t = F.tree_of[v];
   const int p = F.tree_prob[t];
+  if (k >= F.tree_nsites[t]) return;
   if (done && done[p]) return;
-  const int K = F.tree_nsites[t];
   const int sb = F.tree_site_base[t];
   const int c1 = F.child0[v], c2 = F.child1[v];
   const Coef cf1 = F.coef[c1], cf2 = F.coef[c2];
   const double phi = F.phi[p];
-  const bool l1 = c1 < F.num_leaves, l2 = c2 < F.num_leaves;
-  const int64_t rv = F.row_off[v], r1 = F.row_off[c1], r2 = F.row_off[c2];
-  for (int k = lane; k < K; k += W) {
-    const int lv = F.labels[rv + k];
-    int la, lb, Ea, Eb, ca, cb;
-    double a0, aA, b0, bA;
-    load_child(F, l1, r1, k, phi, la, a0, aA, Ea, ca);
-    load_child(F, l2, r2, k, phi, lb, b0, bA, Eb, cb);
-    const int qo = F.site_qoff[sb + k] - 1;
-    const double qa = la > 0 ? F.prior[qo + la] : 1.0;
-    const double qb = lb > 0 ? F.prior[qo + lb] : 1.0;
-    SD e0a, eAa, e0b, eAb;
-    edge_eps(la, a0, aA, Ea, qa, cf1, e0a, eAa);
-    edge_eps(lb, b0, bA, Eb, qb, cf2, e0b, eAb);
-    const SD B0 = sd_mul(e0a, e0b);
-    const SD BA = lv != 0 ? sd_mul(eAa, eAb) : sd_zero();
-    int E = max(sd_top(B0), sd_top(BA));
-    if (E == kZeroTop) E = 0;
-    F.beta[rv + k] = make_double2(ldexp(B0.m, B0.e - E), ldexp(BA.m, BA.e - E));
-    F.bE[rv + k] = E;
-  }
+  const int64_t rv = F.row_off[v];
+  const int lv = F.labels[rv + k];
+  int la, lb, Ea, Eb, ca, cb;
+  double a0, aA, b0, bA;
+  load_child(F, c1 < F.num_leaves, F.row_off[c1], k, phi, la, a0, aA, Ea, ca);
+  load_child(F, c2 < F.num_leaves, F.row_off[c2], k, phi, lb, b0, bA, Eb, cb);
+  const int qo = F.site_qoff[sb + k] - 1;
+  const double qa = la > 0 ? F.prior[qo + la] : 1.0;
+  const double qb = lb > 0 ? F.prior[qo + lb] : 1.0;
+  SD e0a, eAa, e0b, eAb;
+  edge_eps(la, a0, aA, Ea, qa, cf1, e0a, eAa);
+  edge_eps(lb, b0, bA, Eb, qb, cf2, e0b, eAb);
+  const SD B0 = sd_mul(e0a, e0b);
+  const SD BA = lv != 0 ? sd_mul(eAa, eAb) : sd_zero();
+  int E = max(sd_top(B0), sd_top(BA));
+  if (E == kZeroTop) E = 0;
+  F.beta[rv + k] = make_double2(ldexp(B0.m, B0.e - E), ldexp(BA.m, BA.e - E));
+  F.bE[rv + k] = E;
 }
 
 // Root of every tree: site likelihood L = P(D | stem starts unedited), the
-// outside state at the root, and the root branch's posterior counts.
+// outside state at the root, and the root branch's posterior counts (one
+// partial per site chunk).  Segments index trees.
 template <int W>
 __global__ void __launch_bounds__(256) k_root(DevForest F, double* __restrict__ site_llh_out,
-                                              double* __restrict__ counts, const int32_t* __restrict__ done) {
-  const int t = blockIdx.x * (256 / W) + (int)(threadIdx.x / W);
-  const int lane = threadIdx.x & (W - 1);
-  if (t >= F.num_trees) return;
+                                              const int32_t* __restrict__ done) {
+  Seg<W> g;
+  if (!g.init(0, F.num_trees)) return;
+  const int t = g.v, k = g.k, lane = g.lane;
   const int p = F.tree_prob[t];
+  const int K = F.tree_nsites[t];
+  if ((int)blockIdx.y * W >= K) return;
   if (done && done[p]) return;
   const int r = F.tree_root[t];
-  const int K = F.tree_nsites[t];
-  const int sb = F.tree_site_base[t];
-  const Coef cf = F.coef[r];
-  const double phi = F.phi[p];
-  const bool leaf = r < F.num_leaves;
-  const int64_t rr = F.row_off[r];
+  const bool act = k < K;
   double acc0 = 0, acc1 = 0, acc2 = 0;
-  for (int k = lane; k < K; k += W) {
+  if (act) {
+    const int sb = F.tree_site_base[t];
+    const Coef cf = F.coef[r];
+    const double phi = F.phi[p];
+    const bool leaf = r < F.num_leaves;
+    const int64_t rr = F.row_off[r];
     int lab, E, code;
     double b0, bA;
     load_child(F, leaf, rr, k, phi, lab, b0, bA, E, code);
@@ -143,18 +153,18 @@ __global__ void __launch_bounds__(256) k_root(DevForest F, double* __restrict__ 
       F.aG[rr + k] = 0;
     }
     if (lab == 0) {
-      acc0 += 1.0;
+      acc0 = 1.0;
     } else {
-      acc0 += sd_div(sd(cf.p1n * b0, E), L);
-      acc1 += sd_div(sd(cf.mut * q * bA, E), L);
-      if (lab < 0) acc2 += sd_div(sd(cf.sil, 0), L);
+      acc0 = sd_div(sd(cf.p1n * b0, E), L);
+      acc1 = sd_div(sd(cf.mut * q * bA, E), L);
+      if (lab < 0) acc2 = sd_div(sd(cf.sil, 0), L);
     }
   }
   acc0 = seg_sum<W>(acc0);
   acc1 = seg_sum<W>(acc1);
   acc2 = seg_sum<W>(acc2);
-  if (lane == 0 && counts) {
-    double* c = counts + (int64_t)r * 5;
+  if (lane == 0) {
+    double* c = F.cpart + ((int64_t)r * F.max_chunks + blockIdx.y) * 5;
     c[0] = acc0;
     c[1] = acc1;
     c[2] = acc2;
@@ -165,36 +175,38 @@ __global__ void __launch_bounds__(256) k_root(DevForest F, double* __restrict__ 
 
 // Outside pass for the children of the internal nodes of one level, with the
 // posterior expected transition counts of both child branches and the
-// dropout responsibilities of missing leaf children.
+// dropout responsibilities of missing leaf children (one partial per site
+// chunk).
 template <int W>
-__global__ void __launch_bounds__(256) k_outside(DevForest F, int beg, int cnt, double* __restrict__ counts,
-                                                 const int32_t* __restrict__ done) {
-  const int seg = blockIdx.x * (256 / W) + (int)(threadIdx.x / W);
-  const int lane = threadIdx.x & (W - 1);
-  if (seg >= cnt) return;
-  const int u = beg + seg;
+__global__ void __launch_bounds__(256) k_outside(DevForest F, int beg, int cnt, const int32_t* __restrict__ done) {
+  Seg<W> g;
+  if (!g.init(beg, cnt)) return;
+  const int u = g.v, k = g.k, lane = g.lane;
   const int t = F.tree_of[u];
   const int p = F.tree_prob[t];
-  if (done && done[p]) return;
   const int K = F.tree_nsites[t];
-  const int sb = F.tree_site_base[t];
+  if ((int)blockIdx.y * W >= K) return;
+  if (done && done[p]) return;
+  const bool act = k < K;
   const int ch[2] = {F.child0[u], F.child1[u]};
-  const Coef cf[2] = {F.coef[ch[0]], F.coef[ch[1]]};
   const bool lf[2] = {ch[0] < F.num_leaves, ch[1] < F.num_leaves};
-  const int64_t rc[2] = {F.row_off[ch[0]], F.row_off[ch[1]]};
-  const int64_t ru = F.row_off[u];
-  const double phi = F.phi[p];
   double acc[2][5];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 5; ++j) acc[i][j] = 0.0;
   double rt = 0.0;
-  for (int k = lane; k < K; k += W) {
+  if (act) {
+    const int sb = F.tree_site_base[t];
+    const Coef cf[2] = {F.coef[ch[0]], F.coef[ch[1]]};
+    const int64_t rc[2] = {F.row_off[ch[0]], F.row_off[ch[1]]};
+    const int64_t ru = F.row_off[u];
+    const double phi = F.phi[p];
     const double4 au = F.alpha[ru + k];
     const int Eo = F.aE[ru + k];
     const int gu = F.aG[ru + k];
-    const SD L = sd(F.Lm[sb + k], F.LE[sb + k]);
+    // 1/L as a scaled value: posteriors are products with it
+    const SD iL = sd(1.0 / F.Lm[sb + k], -F.LE[sb + k]);
     const int qo = F.site_qoff[sb + k] - 1;
     int lab[2], E[2], code[2];
     double b0[2], bA[2], q[2];
@@ -239,24 +251,28 @@ __global__ void __launch_bounds__(256) k_outside(DevForest F, int beg, int cnt, 
       }
       const int lv = lab[i];
       if (lv == 0) {
-        acc[i][0] += 1.0;
+        acc[i][0] = 1.0;
       } else {
-        const SD bv0 = sd(b0[i], E[i]);
-        const SD bvA = sd(bA[i], E[i]);
-        acc[i][0] += sd_div(sd_mul(av0, bv0), L);
-        const double qv = q[i];
-        acc[i][1] += sd_div(sd_mul(sd_scale(base, qv), bvA), L);
+        // posterior terms share the factor 2^(E_v) / L; mantissas are
+        // multiplied first and the exponent applied once (values are <= 1)
+        const double ms = iL.m;
+        const int es = iL.e + E[i];
+        acc[i][0] = ldexp(av0.m * ms * b0[i], av0.e + es);
+        acc[i][1] = ldexp(base.m * ms * (q[i] * bA[i]), base.e + es);
         SD t3;
         if (lv > 0) {
-          t3 = sd_add(sd_scale(Gy, qv), lv == gg ? Gz : sd_zero());
+          t3 = sd_add(sd_scale(Gy, q[i]), lv == gg ? Gz : sd_zero());
         } else {
           t3 = sumG;
         }
-        acc[i][3] += sd_div(sd_mul(sd_scale(t3, cf[i].pn), bvA), L);
+        acc[i][3] = ldexp(t3.m * ms * (cf[i].pn * bA[i]), t3.e + es);
         if (lv < 0) {
-          acc[i][2] += sd_div(sd_scale(A, cf[i].sil), L);
-          acc[i][4] += sd_div(sd_scale(sumG, cf[i].sil), L);
-          if (lf[i] && code[i] == -1) rt += phi * sd_div(sd_add(av0, sd_add(Yv, Zv)), L);
+          acc[i][2] = ldexp(A.m * iL.m * cf[i].sil, A.e + iL.e);
+          acc[i][4] = ldexp(sumG.m * iL.m * cf[i].sil, sumG.e + iL.e);
+          if (lf[i] && code[i] == -1) {
+            const SD nsil = sd_add(av0, sd_add(Yv, Zv));
+            rt += ldexp(phi * nsil.m * iL.m, nsil.e + iL.e);
+          }
         }
       }
     }
@@ -269,14 +285,25 @@ __global__ void __launch_bounds__(256) k_outside(DevForest F, int beg, int cnt, 
   if (lane == 0) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      if (counts) {
-        double* c = counts + (int64_t)ch[i] * 5;
+      double* c = F.cpart + ((int64_t)ch[i] * F.max_chunks + blockIdx.y) * 5;
 #pragma unroll
-        for (int j = 0; j < 5; ++j) c[j] = acc[i][j];
-      }
+      for (int j = 0; j < 5; ++j) c[j] = acc[i][j];
     }
-    F.rt_part[u] = rt;
+    F.rt_part[(int64_t)u * F.max_chunks + blockIdx.y] = rt;
   }
+}
+
+// Per-node counts: fixed-order sum of the per-chunk partials.
+__global__ void k_counts(DevForest F, double* __restrict__ counts) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= F.num_nodes) return;
+  const int C = (F.tree_nsites[F.tree_of[v]] + F.chunk_w - 1) / F.chunk_w;
+  double s[5] = {0, 0, 0, 0, 0};
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s[j] += F.cpart[((int64_t)v * F.max_chunks + c) * 5 + j];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) counts[(int64_t)v * 5 + j] = s[j];
 }
 
 // ---------------------------------------------------------------------------
@@ -296,7 +323,7 @@ __device__ __forceinline__ double block_sum(double x, double* sh) {
 }
 
 // Sum of per-site log-likelihoods of every tree of a problem, plus the EM
-// bookkeeping of the device loop when em != nullptr.
+// bookkeeping of the device loop when em.done != nullptr.
 __global__ void __launch_bounds__(256) k_llh(DevForest F, double* __restrict__ nll, EmState em, int step,
                                              int max_iter, double tol) {
   __shared__ double sh[4];
@@ -328,18 +355,27 @@ __global__ void __launch_bounds__(256) k_llh(DevForest F, double* __restrict__ n
   }
 }
 
+constexpr int kRedThreads = 1024;
+
+// sum over the problem's internal nodes of the per-node dropout sums
+__device__ __forceinline__ double problem_rtilde(const DevForest& F, int p, double* sh) {
+  double s = 0.0;
+  const int b = F.prob_int_off[p], e = F.prob_node_off[p + 1];
+  const int C = (F.tree_nsites[F.prob_tree_off[p]] + F.chunk_w - 1) / F.chunk_w;
+  for (int i = b + threadIdx.x; i < e; i += kRedThreads) {
+    const int64_t v = F.prob_nodes[i];
+    for (int c = 0; c < C; ++c) s += F.rt_part[v * F.max_chunks + c];
+  }
+  return block_sum<kRedThreads>(s, sh);
+}
+
 // Sum of the dropout responsibilities of missing leaf entries of a problem.
-__global__ void __launch_bounds__(256) k_rtilde(DevForest F, double* __restrict__ rtilde,
-                                                const int32_t* __restrict__ done) {
-  __shared__ double sh[4];
+__global__ void __launch_bounds__(kRedThreads) k_rtilde(DevForest F, double* __restrict__ rtilde,
+                                                        const int32_t* __restrict__ done) {
+  __shared__ double sh[kRedThreads / 64];
   const int p = blockIdx.x;
   if (done && done[p]) return;
-  double s = 0.0;
-  for (int i = F.prob_node_off[p] + threadIdx.x; i < F.prob_node_off[p + 1]; i += 256) {
-    const int v = F.prob_nodes[i];
-    if (v >= F.num_leaves) s += F.rt_part[v];
-  }
-  const double rt = block_sum<256>(s, sh);
+  const double rt = problem_rtilde(F, p, sh);
   if (threadIdx.x == 0) rtilde[p] = rt;
 }
 
@@ -362,8 +398,9 @@ __device__ __forceinline__ void branch_g(const BranchObj& o, double d, double& g
     h -= o.B * o.nu * o.nu * (en + 1.0) / (en * en);
   }
 }
-// argmax over [lo, hi] of a strictly concave objective given its derivative:
-// safeguarded Newton with bisection fallback, to full double precision.
+// argmax over [dmin, dmax] of the strictly concave branch objective.  Its
+// derivative g behaves like S/d - A for small d, so Newton runs on the nearly
+// linear d*g(d) (same root and sign), safeguarded by a bracket.
 __device__ double solve_branch(const BranchObj& o, double dmin, double dmax, double x0) {
   double g, h;
   branch_g(o, dmin, g, h);
@@ -372,122 +409,200 @@ __device__ double solve_branch(const BranchObj& o, double dmin, double dmax, dou
   if (!(g < 0.0)) return dmax;
   double lo = dmin, hi = dmax;
   double x = (x0 > lo && x0 < hi) ? x0 : 0.5 * (lo + hi);
-  for (int it = 0; it < 200; ++it) {
+  for (int it = 0; it < 100; ++it) {
     branch_g(o, x, g, h);
     if (g > 0.0) lo = x; else if (g < 0.0) hi = x; else return x;
-    double xn = x - g / h;
+    const double step = x * g / (g + x * h);
+    // stop once the Newton step is at the rounding-noise level of g
+    if (fabs(step) <= 1e-13 * x) return fmin(fmax(x - step, dmin), dmax);
+    double xn = x - step;
     if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
-    if (fabs(xn - x) <= 4e-16 * x || hi - lo <= 4e-16 * hi) return xn;
+    if (hi - lo <= 1e-13 * hi) return xn;
     x = xn;
   }
   return x;
 }
 
-__global__ void __launch_bounds__(256) k_mstep(DevForest F, const double* __restrict__ counts,
-                                               const double* __restrict__ rtilde, double* __restrict__ d,
-                                               double* __restrict__ nu, double* __restrict__ phi,
-                                               const int32_t* __restrict__ done) {
-  __shared__ double sh[4];
-  __shared__ double s_nu;
-  const int p = blockIdx.x;
+// Branch lengths of every free branch (EM_solver.py:328-366): counts are
+// clamped at eps_s and renormalised to sum K, then the per-branch concave
+// problem is solved exactly with the current nu.  Also stages the nu
+// problem's per-branch coefficients at the branch's position in its problem.
+__global__ void __launch_bounds__(64) k_branch(DevForest F, const double* __restrict__ counts,
+                                                double* __restrict__ d, const double* __restrict__ nu,
+                                                const int32_t* __restrict__ done) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= F.num_nodes) return;
+  const int t = F.tree_of[v];
+  const int p = F.tree_prob[t];
   if (done && done[p]) return;
-  const int nb = F.prob_node_off[p], ne = F.prob_node_off[p + 1];
-  const uint32_t flags = F.prob_flags[p];
-  const double K = (double)F.tree_nsites[F.prob_tree_off[p]];
-  // phi (EM_solver.py:318-326)
-  double phi_star = phi[p];
-  if (flags & 1u) {
-    const double rt = rtilde[p];
-    phi_star = rt / (F.prob_nobs[p] + rt);
-    if (fabs(phi_star) < F.prob_phimin[p]) phi_star = 0.0;
+  const int pos = F.node_pos[v];
+  if (!F.free_branch[v]) {
+    F.nu_a[pos] = 0.0;
+    F.nu_b[pos] = 0.0;
+    F.nu_d[pos] = d[v];
+    return;
   }
-  // branch lengths (EM_solver.py:328-366), exact per-branch maximisation
+  const int Kt = F.tree_nsites[t];
+  const double K = (double)Kt;
+  double s[5];
+  if (counts) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s[j] = counts[(int64_t)v * 5 + j];
+  } else {
+    const int C = (Kt + F.chunk_w - 1) / F.chunk_w;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s[j] = 0.0;
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int j = 0; j < 5; ++j) s[j] += F.cpart[((int64_t)v * F.max_chunks + c) * 5 + j];
+  }
+  double tot = 0.0;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    s[j] = fmax(kEpsS, s[j]);
+    tot += s[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 5; ++j) s[j] = s[j] / tot * K;
   const double nu0 = nu[p];
-  const bool use_nu = nu0 > kEpsNu;
-  double nfree = 0.0;
-  for (int i = nb + threadIdx.x; i < ne; i += 256) {
-    const int v = F.prob_nodes[i];
-    if (!F.free_branch[v]) continue;
-    const double* c = counts + (int64_t)v * 5;
-    double s[5];
-    double tot = 0.0;
-#pragma unroll
-    for (int j = 0; j < 5; ++j) {
-      s[j] = fmax(kEpsS, c[j]);
-      tot += s[j];
-    }
-#pragma unroll
-    for (int j = 0; j < 5; ++j) s[j] = s[j] / tot * K;
-    double* sn = F.snorm + (int64_t)v * 5;
-#pragma unroll
-    for (int j = 0; j < 5; ++j) sn[j] = s[j];
-    BranchObj o;
-    o.nu = nu0;
-    o.A = (nu0 + 1.0) * s[0] + nu0 * s[1] + nu0 * s[3];
-    o.S1 = s[1];
-    o.B = use_nu ? (s[2] + s[4]) : 0.0;
-    d[v] = solve_branch(o, F.dmin, F.dmax, d[v]);
-    nfree += 1.0;
+  BranchObj o;
+  o.nu = nu0;
+  o.A = (nu0 + 1.0) * s[0] + nu0 * s[1] + nu0 * s[3];
+  o.S1 = s[1];
+  o.B = nu0 > kEpsNu ? (s[2] + s[4]) : 0.0;
+  const double dn = solve_branch(o, F.dmin, F.dmax, d[v]);
+  d[v] = dn;
+  F.nu_a[pos] = s[0] + s[1] + s[3];
+  F.nu_b[pos] = s[2] + s[4];
+  F.nu_d[pos] = dn;
+}
+
+// phi closed form (EM_solver.py:318-326) and nu (EM_solver.py:392-407).
+// nu maximises a concave function of one variable over [0, 1] given the new
+// branch lengths; its derivative h'(x) = sum_v d_v (b_v / expm1(x d_v) - a_v)
+// sums over every branch of the problem.  Each problem's branches are split
+// over several blocks; one launch = one Newton evaluation: every block writes
+// its partial (h', h''), and the last block of the problem to arrive reduces
+// the partials in block order (deterministic) and takes a bracketed Newton
+// step.  The first launch also sums the dropout responsibilities for phi.
+constexpr int kNuThreads = 256;
+constexpr int kNuIters = 8;
+
+__global__ void __launch_bounds__(kNuThreads) k_nu_eval(DevForest F, const double* __restrict__ rtilde_in,
+                                                        double* __restrict__ nu, double* __restrict__ phi,
+                                                        const int32_t* __restrict__ done, int iter) {
+  __shared__ double sh[kNuThreads / 64];
+  __shared__ int s_last;
+  const int blk = blockIdx.x;
+  // block -> (problem, part) by binary search over prob_blk_off
+  int lo_p = 0, hi_p = F.num_problems;
+  while (hi_p - lo_p > 1) {
+    const int mid = (lo_p + hi_p) >> 1;
+    if (F.prob_blk_off[mid] <= blk) lo_p = mid; else hi_p = mid;
   }
-  nfree = block_sum<256>(nfree, sh);
+  const int p = lo_p;
+  if (done && done[p]) return;
+  NuState* st = F.nu_state + p;
+  const uint32_t flags = F.prob_flags[p];
+  const bool opt_nu = (flags & 2u) && F.prob_nfree[p] > 0;
+  if (iter > 0 && (!opt_nu || st->fin)) return;
+  const int part = blk - F.prob_blk_off[p];
+  const int nparts = F.prob_blk_off[p + 1] - F.prob_blk_off[p];
+  const int b = F.prob_node_off[p], e = F.prob_node_off[p + 1];
+  const int span = (e - b + nparts - 1) / nparts;
+  const int pb = b + part * span, pe = min(e, pb + span);
+  // first evaluation point: the current nu (or 1/2 when outside (0, 1))
+  const double x = (iter == 0) ? ((nu[p] > 0.0 && nu[p] < 1.0) ? nu[p] : 0.5) : st->x;
+  double g = 0.0, h = 0.0, rt = 0.0;
+  if (opt_nu) {
+    for (int i = pb + threadIdx.x; i < pe; i += kNuThreads) {
+      const double bb = F.nu_b[i];
+      if (bb == 0.0) continue;
+      const double dv = F.nu_d[i];
+      const double en = expm1(x * dv);
+      g += dv * (bb / en - F.nu_a[i]);
+      h -= bb * dv * dv * (en + 1.0) / (en * en);
+    }
+  }
+  if (iter == 0 && (flags & 1u) && !rtilde_in) {
+    const int C = (F.tree_nsites[F.prob_tree_off[p]] + F.chunk_w - 1) / F.chunk_w;
+    for (int i = max(pb, F.prob_int_off[p]) + threadIdx.x; i < pe; i += kNuThreads) {
+      const int64_t v = F.prob_nodes[i];
+      for (int c = 0; c < C; ++c) rt += F.rt_part[v * F.max_chunks + c];
+    }
+  }
+  g = block_sum<kNuThreads>(g, sh);
+  h = block_sum<kNuThreads>(h, sh);
+  rt = block_sum<kNuThreads>(rt, sh);
+  if (threadIdx.x == 0) {
+    double* pp = F.nu_part + (int64_t)blk * 3;
+    pp[0] = g;
+    pp[1] = h;
+    pp[2] = rt;
+    __threadfence();
+    const int prev = atomicAdd(&st->count, 1);
+    s_last = (prev == nparts - 1);
+  }
   __syncthreads();
-  // nu (EM_solver.py:392-407): maximise over [0, 1] given the new lengths
-  if ((flags & 2u) && nfree > 0.0) {
-    // h'(x) = sum_v -d (S0+S1+S3) + (S2+S4) d / expm1(x d)
-    double lo = 0.0, hi = 1.0;
-    double x = 1.0;
-    bool fin = false;
-    for (int it = 0; it < 400 && !fin; ++it) {
-      double g = 0.0, h = 0.0;
-      for (int i = nb + threadIdx.x; i < ne; i += 256) {
-        const int v = F.prob_nodes[i];
-        if (!F.free_branch[v]) continue;
-        const double* s = F.snorm + (int64_t)v * 5;
-        const double dv = d[v];
-        const double en = expm1(x * dv);
-        const double B = s[2] + s[4];
-        g += -dv * (s[0] + s[1] + s[3]) + B * dv / en;
-        h -= B * dv * dv * (en + 1.0) / (en * en);
-      }
-      g = block_sum<256>(g, sh);
-      h = block_sum<256>(h, sh);
-      if (it == 0) {
-        if (!(g < 0.0)) {  // optimum at the upper bound
-          x = 1.0;
-          break;
-        }
-        hi = 1.0;
-        x = (nu0 > 0.0 && nu0 < 1.0) ? nu0 : 0.5;
-        continue;
-      }
-      if (g > 0.0) lo = x; else if (g < 0.0) hi = x; else break;
-      double xn = x - g / h;
-      if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
-      if (fabs(xn - x) <= 4e-16 * x || hi - lo <= 4e-16 * hi) {
-        x = xn;
-        fin = true;
-      } else {
-        x = xn;
-      }
-    }
-    if (threadIdx.x == 0) s_nu = x;
-    __syncthreads();
-    if (threadIdx.x == 0) nu[p] = s_nu;
+  if (!s_last || threadIdx.x != 0) return;
+  __threadfence();
+  st->count = 0;
+  double G = 0.0, Hs = 0.0, RT = 0.0;
+  for (int q = F.prob_blk_off[p]; q < F.prob_blk_off[p + 1]; ++q) {
+    const volatile double* pp = F.nu_part + (int64_t)q * 3;
+    G += pp[0];
+    Hs += pp[1];
+    RT += pp[2];
   }
-  if (threadIdx.x == 0) phi[p] = phi_star;
+  if (iter == 0) {
+    double phi_star = phi[p];
+    if (flags & 1u) {
+      const double r = rtilde_in ? rtilde_in[p] : RT;
+      phi_star = r / (F.prob_nobs[p] + r);
+      if (fabs(phi_star) < F.prob_phimin[p]) phi_star = 0.0;
+    }
+    phi[p] = phi_star;
+    st->lo = 0.0;
+    st->hi = 2.0;  // > 1: the bound x = 1 has not been evaluated
+    st->fin = opt_nu ? 0 : 1;
+    if (!opt_nu) return;
+  }
+  // bracketed Newton step on x h'(x), which is nearly linear in x
+  double lo = st->lo, hi = st->hi;
+  if (G > 0.0) lo = x; else if (G < 0.0) hi = x;
+  double xn;
+  int fin = 0;
+  if (G > 0.0 && x >= 1.0) {
+    xn = 1.0;  // optimum at the upper bound
+    fin = 1;
+  } else if (G == 0.0) {
+    xn = x;
+    fin = 1;
+  } else {
+    const double step = x * G / (G + x * Hs);
+    const double top = fmin(hi, 1.0);
+    // the sum over branches carries ~1e-14 relative rounding noise near the
+    // root; a Newton step below 1e-12 x is converged
+    if (fabs(step) <= 1e-12 * x) {
+      xn = fmin(fmax(x - step, 0.0), 1.0);
+      fin = 1;
+    } else {
+      xn = x - step;
+      if (xn >= 1.0 && hi > 1.0) xn = 1.0;
+      else if (!(xn > lo && xn < top)) xn = 0.5 * (lo + top);
+      fin = (top - lo <= 1e-12 * top) ? 1 : 0;
+    }
+  }
+  st->lo = lo;
+  st->hi = hi;
+  st->x = xn;
+  st->fin = fin;
+  nu[p] = xn;
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int W>
-static void launch_seg(void (*kern)(DevForest, int, int, const int32_t*), const DevForest& F, int beg, int cnt,
-                       const int32_t* done, hipStream_t s) {
-  const int segs = 256 / W;
-  const int grid = (cnt + segs - 1) / segs;
-  if (grid > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, F, beg, cnt, done);
-}
-
 int seg_width(int max_k) {
   if (max_k <= 8) return 8;
   if (max_k <= 16) return 16;
@@ -495,20 +610,25 @@ int seg_width(int max_k) {
   return 64;
 }
 
-#define LAML_DISPATCH_W(W, BODY)         \
-  switch (W) {                           \
+#define LAML_DISPATCH_W(W, BODY)                     \
+  switch (W) {                                       \
     case 8: { constexpr int WW = 8; BODY; } break;   \
     case 16: { constexpr int WW = 16; BODY; } break; \
     case 32: { constexpr int WW = 32; BODY; } break; \
     default: { constexpr int WW = 64; BODY; } break; \
   }
 
+static dim3 seg_grid(int cnt, int W, int chunks) {
+  const int segs = 256 / W;
+  return dim3((cnt + segs - 1) / segs, chunks);
+}
+
 void run_labels(const DevForest& F, const std::vector<int32_t>& lvl, int W, hipStream_t s) {
   for (size_t h = 1; h + 1 < lvl.size(); ++h) {
     const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
+    if (cnt <= 0) continue;
     LAML_DISPATCH_W(W, {
-      const int grid = (cnt + 256 / WW - 1) / (256 / WW);
-      if (grid > 0) hipLaunchKernelGGL(k_labels<WW>, dim3(grid), dim3(256), 0, s, F, beg, cnt);
+      hipLaunchKernelGGL(k_labels<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt);
     });
   }
 }
@@ -521,26 +641,32 @@ void run_coef(const DevForest& F, const double* d, const double* nu, const int32
 void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, const int32_t* done, hipStream_t s) {
   for (size_t h = 1; h + 1 < lvl.size(); ++h) {
     const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
-    LAML_DISPATCH_W(W, { launch_seg<WW>(k_inside<WW>, F, beg, cnt, done, s); });
+    if (cnt <= 0) continue;
+    LAML_DISPATCH_W(W, {
+      hipLaunchKernelGGL(k_inside<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt, done);
+    });
   }
 }
 
-void run_root(const DevForest& F, int W, double* site_llh_out, double* counts, const int32_t* done, hipStream_t s) {
+void run_root(const DevForest& F, int W, double* site_llh_out, const int32_t* done, hipStream_t s) {
   LAML_DISPATCH_W(W, {
-    const int grid = (F.num_trees + 256 / WW - 1) / (256 / WW);
-    hipLaunchKernelGGL(k_root<WW>, dim3(grid), dim3(256), 0, s, F, site_llh_out, counts, done);
+    hipLaunchKernelGGL(k_root<WW>, seg_grid(F.num_trees, WW, F.max_chunks), dim3(256), 0, s, F, site_llh_out,
+                       done);
   });
 }
 
-void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, double* counts,
-                 const int32_t* done, hipStream_t s) {
+void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, const int32_t* done, hipStream_t s) {
   for (size_t h = lvl.size() - 2; h >= 1; --h) {
     const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
+    if (cnt <= 0) continue;
     LAML_DISPATCH_W(W, {
-      const int grid = (cnt + 256 / WW - 1) / (256 / WW);
-      if (grid > 0) hipLaunchKernelGGL(k_outside<WW>, dim3(grid), dim3(256), 0, s, F, beg, cnt, counts, done);
+      hipLaunchKernelGGL(k_outside<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt, done);
     });
   }
+}
+
+void run_counts(const DevForest& F, double* counts, hipStream_t s) {
+  hipLaunchKernelGGL(k_counts, dim3((F.num_nodes + 255) / 256), dim3(256), 0, s, F, counts);
 }
 
 void run_llh(const DevForest& F, double* nll, EmState em, int step, int max_iter, double tol, hipStream_t s) {
@@ -548,12 +674,14 @@ void run_llh(const DevForest& F, double* nll, EmState em, int step, int max_iter
 }
 
 void run_rtilde(const DevForest& F, double* rtilde, const int32_t* done, hipStream_t s) {
-  hipLaunchKernelGGL(k_rtilde, dim3(F.num_problems), dim3(256), 0, s, F, rtilde, done);
+  hipLaunchKernelGGL(k_rtilde, dim3(F.num_problems), dim3(kRedThreads), 0, s, F, rtilde, done);
 }
 
 void run_mstep(const DevForest& F, const double* counts, const double* rtilde, double* d, double* nu, double* phi,
                const int32_t* done, hipStream_t s) {
-  hipLaunchKernelGGL(k_mstep, dim3(F.num_problems), dim3(256), 0, s, F, counts, rtilde, d, nu, phi, done);
+  hipLaunchKernelGGL(k_branch, dim3((F.num_nodes + 63) / 64), dim3(64), 0, s, F, counts, d, nu, done);
+  for (int it = 0; it <= kNuIters; ++it)
+    hipLaunchKernelGGL(k_nu_eval, dim3(F.num_nu_blocks), dim3(kNuThreads), 0, s, F, rtilde, nu, phi, done, it);
 }
 
 }  // namespace laml

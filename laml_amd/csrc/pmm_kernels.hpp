@@ -9,9 +9,18 @@
 
 namespace laml {
 
+// Newton state of one problem's nu solve
+struct NuState {
+  double x, lo, hi;
+  int32_t fin;
+  int32_t count;  // blocks arrived in the current evaluation
+};
+
 // Device view of a forest; passed by value to every kernel.
 struct DevForest {
   int32_t num_nodes, num_leaves, num_trees, num_problems;
+  int32_t chunk_w;     // sites per segment (W)
+  int32_t max_chunks;  // ceil(max K / W)
   double dmin, dmax;
   // topology (level order)
   const int32_t* child0;
@@ -26,7 +35,12 @@ struct DevForest {
   const int32_t* tree_site_base;
   const int32_t* prob_tree_off;  // [P+1]
   const int32_t* prob_node_off;  // [P+1]
-  const int32_t* prob_nodes;     // nodes grouped by problem
+  const int32_t* prob_nodes;     // nodes grouped by problem (leaves first)
+  const int32_t* prob_int_off;   // [P] first internal node of each problem in prob_nodes
+  const int32_t* prob_nfree;     // [P] number of optimised branches
+  const int32_t* node_pos;       // [N] position of each node in prob_nodes
+  const int32_t* prob_blk_off;   // [P+1] blocks of the nu solve per problem
+  int32_t num_nu_blocks;
   const uint32_t* prob_flags;
   const double* prob_nobs;
   const double* prob_phimin;
@@ -47,8 +61,13 @@ struct DevForest {
   double* Lm;  // [sites]
   int32_t* LE;
   double* site_llh;
-  double* rt_part;  // [N]
-  double* snorm;    // [N*5]
+  double* rt_part;  // [N * max_chunks] dropout sums of leaf children per chunk
+  double* cpart;    // [N * max_chunks * 5] per-chunk branch count partials
+  double* nu_a;     // [N] by position in prob_nodes: S0+S1+S3 of the branch
+  double* nu_b;     // [N] S2+S4 (0 for fixed branches)
+  double* nu_d;     // [N] new branch length
+  double* nu_part;  // [num_nu_blocks * 3] partial sums of the nu solve
+  NuState* nu_state;  // [P]
 };
 
 struct EmState {
@@ -62,9 +81,9 @@ int seg_width(int max_k);
 void run_labels(const DevForest& F, const std::vector<int32_t>& lvl, int W, hipStream_t s);
 void run_coef(const DevForest& F, const double* d, const double* nu, const int32_t* done, hipStream_t s);
 void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, const int32_t* done, hipStream_t s);
-void run_root(const DevForest& F, int W, double* site_llh_out, double* counts, const int32_t* done, hipStream_t s);
-void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, double* counts, const int32_t* done,
-                 hipStream_t s);
+void run_root(const DevForest& F, int W, double* site_llh_out, const int32_t* done, hipStream_t s);
+void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, const int32_t* done, hipStream_t s);
+void run_counts(const DevForest& F, double* counts, hipStream_t s);
 void run_llh(const DevForest& F, double* nll, EmState em, int step, int max_iter, double tol, hipStream_t s);
 void run_rtilde(const DevForest& F, double* rtilde, const int32_t* done, hipStream_t s);
 void run_mstep(const DevForest& F, const double* counts, const double* rtilde, double* d, double* nu, double* phi,

@@ -51,10 +51,19 @@ def load_test1():
 
 
 def load_example1():
-    """examples/example1 (250 cells x 30 sites, the shape of the 250x30 config)."""
+    """examples/example1 (250 cells x 30 sites, the shape of the 250x30 config).
+    The starting tree carries no branch lengths; seeded lengths in
+    [0.01, 0.3] are attached so the likelihood is defined."""
+    import random
+
+    from laml_amd.tree import read_tree_newick
     from oracle.em_oracle import read_charmtrx
 
-    T = open(golden_path("example1", "starting.tree")).read()
+    t = read_tree_newick(open(golden_path("example1", "starting.tree")).read())
+    rng = random.Random(250)
+    for v in t.traverse_postorder():
+        v.edge_length = round(rng.uniform(0.01, 0.3), 6)
+    T = t.newick()
     msa = read_charmtrx(golden_path("example1", "character_matrix.csv"))
     k = len(next(iter(msa.values())))
     Q = [{} for _ in range(k)]
