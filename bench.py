@@ -1,14 +1,17 @@
 """Benchmark: fixed-topology PMM EM on one MI355X.
 
-Workload (default): a seeded 10,000-leaf Yule tree (height 1) x 300 sites,
-alphabet sizes uniform in [5, 60], Dirichlet(1) priors, characters simulated
-with nu = 0.1, phi = 0.1; EM from a uniform start (all branches 0.05,
-nu = phi = 0.5), every step one full EM iteration (inside pass, outside pass
-with posterior counts, device M-step of branch lengths, nu and phi) in FP64.
+Workload (default): B = 16 independent problems resident in HBM, each a
+seeded 10,000-leaf Yule tree (height 1) x 300 sites, alphabet sizes uniform
+in [5, 60], Dirichlet(1) priors, characters simulated with nu = 0.1,
+phi = 0.1; EM from a uniform start (all branches 0.05, nu = phi = 0.5).  One
+step is one full EM iteration of every problem of the batch (inside pass,
+outside pass with posterior counts, device M-step of branch lengths, nu and
+phi), all FP64, in shared level-synchronous launches.
 
-One JSON line on stdout: EM iterations/s as ``value``, plus node-site
-evaluations/s, the HBM roofline of the dominant kernel (the outside pass) and
-a CPU baseline (the oracle on a bounded sample of the same workload).
+One JSON line on stdout: aggregate EM iterations/s as ``value`` (B x steps /
+time), the same for a single problem alone, node-site evaluations/s, the HBM
+roofline of the dominant kernel family (the outside pass) and a CPU baseline
+(the oracle on a bounded sample of the same workload).
 """
 import argparse
 import json
@@ -106,6 +109,7 @@ def main():
     ap.add_argument("--workload", default="em10k", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-reps", type=int, default=10)
+    ap.add_argument("--batch", type=int, default=16, help="independent problems per step")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the PMM path on a single MI355X "
@@ -118,36 +122,52 @@ def main():
     from laml_amd.forest import Forest
 
     n, K, mlo, mhi, alpha, nu_true, phi_true, seed = WORKLOADS[args.workload]
-    log("generating workload %s: n=%d K=%d" % (args.workload, n, K))
-    tree, Q, codes, truth = sim.make_problem(n, K, mlo, mhi, alpha, nu_true, phi_true, seed)
-    spec = sim.problem_spec(tree, Q, codes)
+    B = max(1, args.batch)
+    log("generating workload %s: %d problems of n=%d K=%d" % (args.workload, B, n, K))
+    probs = [sim.make_problem(n, K, mlo, mhi, alpha, nu_true, phi_true, seed + 1000 * b) for b in range(B)]
+    tree, Q, codes, truth = probs[0]
     torch.cuda.set_device(0)
-    t0 = time.perf_counter()
-    F = Forest([spec], device=0)
-    torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t0
-    dev = F.device
-    d0 = F.lengths_to_level([np.full(tree.n_nodes, 0.05)])
-    d = torch.tensor(d0, dtype=torch.float64, device=dev)
-    nu = torch.tensor([0.5], dtype=torch.float64, device=dev)
-    phi = torch.tensor([0.5], dtype=torch.float64, device=dev)
-    log("forest ready in %.2f s (N=%d, levels=%d); warmup %d EM iterations" % (setup_s, F.num_nodes,
-                                                                               F.num_levels, args.warmup))
-    if args.warmup > 0:
-        F.em(d, nu, phi, args.warmup, 0.0)
-    torch.cuda.synchronize()
-    log("timing %d EM iterations" % args.steps)
-    t0 = time.perf_counter()
-    nll, _ = F.em(d, nu, phi, args.steps, 0.0)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    em_per_s = args.steps / el
-    final_nll = float(nll.item())
-    log("EM: %.3f ms/iter, %.1f it/s, nllh %.6f" % (el / args.steps * 1e3, em_per_s, final_nll))
+
+    def run(problems, steps, warmup):
+        specs = [sim.problem_spec(t, q, c) for (t, q, c, _) in problems]
+        t0 = time.perf_counter()
+        F = Forest(specs, device=0)
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        dev = F.device
+        d0 = F.lengths_to_level([np.full(t.n_nodes, 0.05) for (t, _, _, _) in problems])
+        d = torch.tensor(d0, dtype=torch.float64, device=dev)
+        nu = torch.full((len(problems),), 0.5, dtype=torch.float64, device=dev)
+        phi = torch.full((len(problems),), 0.5, dtype=torch.float64, device=dev)
+        if warmup > 0:
+            F.em(d, nu, phi, warmup, 0.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nll, _ = F.em(d, nu, phi, steps, 0.0)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        return F, d, nu, phi, nll, el, setup
+
+    # single problem alone (the plain per-tree GPU path)
+    F1, d1, nu1, phi1, nll1, el1, _ = run(probs[:1], args.steps, args.warmup)
+    single_per_s = args.steps / el1
+    single_nll = float(nll1[0].item())
+    log("single problem: %.3f ms/iter, %.1f EM it/s, nllh %.6f" % (el1 / args.steps * 1e3, single_per_s, single_nll))
+    F1.close()
+    del F1, d1, nu1, phi1
+    # the batch
+    F, d, nu, phi, nll, el, setup_s = run(probs, args.steps, args.warmup)
+    em_per_s = B * args.steps / el
+    final_nll = float(nll[0].item())
+    log("batch of %d: %.3f ms/iter, %.1f aggregate EM it/s (x%.1f the single-problem rate); setup %.2f s" % (
+        B, el / args.steps * 1e3, em_per_s, em_per_s / single_per_s, setup_s))
+    if abs(final_nll - single_nll) > 1e-9 * abs(single_nll):
+        raise SystemExit("batched result differs from the single-problem result: %r vs %r" % (final_nll, single_nll))
 
     # roofline of the dominant kernel family: the outside pass (all level
-    # launches of k_outside + the root and dropout-sum kernels), timed with
-    # events on the launch stream over resident inside state
+    # launches of k_outside, the fused top-level kernel, k_root, k_counts,
+    # k_rtilde), timed with events on the launch stream over resident inside
+    # state
     inside_b, outside_b = algorithmic_bytes(F)
     F.nll(d, nu, phi)
     stream = torch.cuda.current_stream()
@@ -170,7 +190,7 @@ def main():
     log("outside pass %.3f ms (%.1f GB/s), inside pass %.3f ms (%.1f GB/s)" % (
         out_ms, achieved, in_ms, inside_b / (in_ms * 1e-3) / 1e9))
 
-    node_site = 2 * F.num_nodes * K
+    node_site = 2 * F.num_nodes * K  # over the whole batch
     res = {
         "metric": "EM iters/s at 10k cells x 300 sites",
         "value": round(em_per_s, 3),
@@ -179,17 +199,19 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(el / args.steps * 1e3, 4),
+        "single_problem": {"em_iters_per_s": round(single_per_s, 3), "ms_per_iter": round(el1 / args.steps * 1e3, 4),
+                           "batch_speedup": round(em_per_s / single_per_s, 2)},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded Yule tree + PMM-simulated characters)",
-        "config": {"workload": "fixed-topology PMM EM, %d-leaf Yule tree x %d sites, M in [%d,%d], "
-                               "Dirichlet(%g), nu=%g, phi=%g, seed %d" % (n, K, mlo, mhi, alpha, nu_true, phi_true,
-                                                                          seed),
-                   "leaves": n, "sites": K, "nodes": F.num_nodes, "levels": F.num_levels},
+        "config": {"workload": "fixed-topology PMM EM, batch of %d independent problems, each a %d-leaf Yule "
+                               "tree x %d sites, M in [%d,%d], Dirichlet(%g), nu=%g, phi=%g, seeds %d+1000*b"
+                               % (B, n, K, mlo, mhi, alpha, nu_true, phi_true, seed),
+                   "batch": B, "leaves": n, "sites": K, "nodes": F.num_nodes, "levels": F.num_levels},
         "node_site_evals_per_s": round(node_site * em_per_s, 1),
-        "final_nllh": final_nll,
+        "final_nllh_problem0": final_nll,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "kernel": "outside pass (k_outside levels + k_root + k_rtilde)",

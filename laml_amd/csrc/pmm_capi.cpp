@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -33,6 +34,7 @@ int fail(int code, const std::string& msg) {
 struct laml_forest {
   int device = 0;
   int W = 64;
+  int h_top = 1 << 30;  // first level handled by the fused top-level kernels
   DevForest F{};
   std::vector<int32_t> level_off;  // host copy
   std::vector<int32_t> tree_nsites;
@@ -248,6 +250,14 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   f->W = laml::seg_width(max_k);
   f->level_off.assign(d->level_off, d->level_off + d->num_levels + 1);
   f->tree_nsites.assign(d->tree_nsites, d->tree_nsites + d->num_trees);
+  // fuse the tail of small upper levels into one launch per pass
+  {
+    const char* env = getenv("LAML_TOP_MAX_NODES");
+    const int top_max = env ? atoi(env) : 48;
+    int ht = d->num_levels;
+    while (ht - 1 >= 1 && d->level_off[ht] - d->level_off[ht - 1] <= top_max) --ht;
+    f->h_top = ht;
+  }
   f->leaf_rows = leaf_rows;
   f->int_rows = int_rows;
   const int N = d->num_nodes, T = d->num_trees, P = d->num_problems;
@@ -265,6 +275,7 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   std::vector<int32_t> tree_prob(d->tree_problem, d->tree_problem + T);
 #define UP(dst, src, n) \
   if ((rc = f->upload(&F.dst, src, (size_t)(n)))) { delete f; return rc; }
+  UP(level_off, d->level_off, d->num_levels + 1);
   UP(child0, d->child0, N);
   UP(child1, d->child1, N);
   UP(tree_of, d->tree_of, N);
@@ -407,7 +418,7 @@ int laml_nll(laml_forest* f, const double* d, const double* nu, const double* ph
   DevForest F = f->F;
   F.phi = phi;
   laml::run_coef(F, d, nu, nullptr, s);
-  laml::run_inside(F, f->level_off, f->W, nullptr, s);
+  laml::run_inside(F, f->level_off, f->W, f->h_top, nullptr, s);
   laml::run_root(F, f->W, site_llh, nullptr, s);
   EmState none{};
   laml::run_llh(F, nll, none, 0, 0, 0.0, s);
@@ -422,11 +433,11 @@ int laml_estep(laml_forest* f, const double* d, const double* nu, const double* 
   DevForest F = f->F;
   F.phi = phi;
   laml::run_coef(F, d, nu, nullptr, s);
-  laml::run_inside(F, f->level_off, f->W, nullptr, s);
+  laml::run_inside(F, f->level_off, f->W, f->h_top, nullptr, s);
   laml::run_root(F, f->W, nullptr, nullptr, s);
   EmState none{};
   laml::run_llh(F, nll, none, 0, 0, 0.0, s);
-  laml::run_outside(F, f->level_off, f->W, nullptr, s);
+  laml::run_outside(F, f->level_off, f->W, f->h_top, nullptr, s);
   laml::run_counts(F, counts, s);
   laml::run_rtilde(F, rtilde, nullptr, s);
   return check_launch();
@@ -439,7 +450,7 @@ int laml_outside(laml_forest* f, const double* phi, double* counts, double* rtil
   DevForest F = f->F;
   F.phi = phi;
   laml::run_root(F, f->W, nullptr, nullptr, s);
-  laml::run_outside(F, f->level_off, f->W, nullptr, s);
+  laml::run_outside(F, f->level_off, f->W, f->h_top, nullptr, s);
   laml::run_counts(F, counts, s);
   laml::run_rtilde(F, rtilde, nullptr, s);
   return check_launch();
@@ -471,11 +482,11 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
   const int chunk = 8;
   for (int step = 0; step <= max_iter; ++step) {
     laml::run_coef(F, d, nu, em.done, s);
-    laml::run_inside(F, f->level_off, f->W, em.done, s);
+    laml::run_inside(F, f->level_off, f->W, f->h_top, em.done, s);
     laml::run_root(F, f->W, nullptr, em.done, s);
     laml::run_llh(F, nll, em, step, max_iter, tol, s);
     if (step < max_iter) {
-      laml::run_outside(F, f->level_off, f->W, em.done, s);
+      laml::run_outside(F, f->level_off, f->W, f->h_top, em.done, s);
       laml::run_mstep(F, nullptr, nullptr, d, nu, phi, em.done, s);
     }
     int rc = check_launch();

@@ -52,9 +52,10 @@ __device__ __forceinline__ double sd_log(SD a) {
 //   p1n = P(0->0) = e^{-(1+nu)d}
 //   mut = P(0->a)/q_a = (1-e^{-d}) e^{-nu d}
 //   pn  = P(a->a) = e^{-nu d}
-//   sil = P(0->-1) = P(a->-1) = 1-e^{-nu d}
+//   sil = P(0->-1) = P(a->-1) = 1-e^{-nu d} = msil * 2^esil (msil in [0.5, 1) or 0)
 struct Coef {
-  double p1n, mut, pn, sil;
+  double p1n, mut, pn, msil;
+  int esil, pad;
 };
 
 // label codes: 0 = 'z', -1 = '?', j>0 = edit state j of the site
@@ -89,22 +90,57 @@ __device__ __forceinline__ void leaf_beta(int code, double phi, int& lab, double
   }
 }
 
-// edge-inside of a child seen from its parent: e0 = P(D_c | parent 0),
-// eA = P(D_c | parent in the child's edit state) ('?' child: any edit).
-// q is the prior of the child's specific state (1 for '?').
-__device__ __forceinline__ void edge_eps(int lab, double b0, double bA, int E, double q, const Coef& c,
-                                         SD& e0, SD& eA) {
-  if (lab == 0) {
-    e0 = sd(c.p1n * b0, E);
-    eA = sd_zero();
-    return;
-  }
-  e0 = sd(fma(c.mut * q, bA, c.p1n * b0), E);
-  eA = sd(c.pn * bA, E);
-  if (lab < 0) {
-    e0 = sd_add(e0, sd(c.sil, 0));
-    eA = sd_add(eA, sd(c.sil, 0));
-  }
+// Edge-inside of a child seen from its parent, both values on one exponent:
+// eps(0) = e0 2^e = P(D_c | parent 0); eps(alpha) = eA 2^e = P(D_c | parent in
+// the child's edit state) ('?' child: any edit state).  The child's inside
+// vector is (b0, bA) 2^E with beta(-1) = 1 for '?' children; q is the prior
+// of the child's specific state.  Branch-free over the three label kinds.
+struct Eps {
+  double e0, eA;
+  int e;
+};
+__device__ __forceinline__ Eps edge(int lab, double b0, double bA, int E, double q, const Coef& c) {
+  const bool z = lab == 0;
+  const double t0 = fma(z ? 0.0 : c.mut * q, bA, c.p1n * b0);
+  const double tA = z ? 0.0 : c.pn * bA;
+  const int es = (lab < 0 && c.msil != 0.0) ? c.esil : kZeroTop;
+  const int e = max(E, es);
+  const double s0 = ldexp(1.0, E - e);
+  const double s1 = ldexp(c.msil, es - e);
+  Eps r;
+  r.e0 = fma(t0, s0, s1);
+  r.eA = fma(tA, s0, s1);
+  r.e = e;
+  return r;
+}
+
+// binary exponent x of m (m in [2^(x-1), 2^x)); 0 for m == 0
+__device__ __forceinline__ int exp_of(double m) {
+  int ex;
+  (void)frexp(m, &ex);
+  return ex;
+}
+
+// 64-bit DPP move of a double (rows not in ROW_MASK read 0)
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_mov(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW_MASK, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over an aligned segment of W lanes (W = 8..64) with DPP: quad
+// permutations, row rotations and row broadcasts.  The sum is valid in the
+// LAST lane of each segment.
+template <int W>
+__device__ __forceinline__ double seg_sum_last(double x) {
+  x += dpp_mov<0xb1>(x);  // quad_perm [1,0,3,2]
+  x += dpp_mov<0x4e>(x);  // quad_perm [2,3,0,1]
+  if (W >= 8) x += dpp_mov<0x124>(x);  // row_ror:4
+  if (W >= 16) x += dpp_mov<0x128>(x);  // row_ror:8
+  if (W >= 32) x += dpp_mov<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+  if (W >= 64) x += dpp_mov<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+  return x;
 }
 
 template <int W>

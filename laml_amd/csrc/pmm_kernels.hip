@@ -13,6 +13,8 @@
 // per-node reductions over sites are segment shuffles.
 #include "pmm_kernels.hpp"
 
+#include <algorithm>
+
 namespace laml {
 
 __global__ void k_coef(DevForest F, const double* __restrict__ d, const double* __restrict__ nu,
@@ -27,7 +29,10 @@ __global__ void k_coef(DevForest F, const double* __restrict__ d, const double* 
   c.p1n = exp(-(1.0 + nv) * dv);
   c.pn = exp(-nv * dv);
   c.mut = -expm1(-dv) * c.pn;
-  c.sil = -expm1(-nv * dv);
+  int es;
+  c.msil = frexp(-expm1(-nv * dv), &es);
+  c.esil = c.msil != 0.0 ? es : 0;
+  c.pad = 0;
   F.coef[v] = c;
 }
 
@@ -77,11 +82,8 @@ __device__ __forceinline__ void load_child(const DevForest& F, bool leaf, int64_
   }
 }
 
-template <int W>
-__global__ void __launch_bounds__(256) k_inside(DevForest F, int beg, int cnt, const int32_t* __restrict__ done) {
-  Seg<W> g;
-  if (!g.init(beg, cnt)) return;
-  const int v = g.v, k = g.k;
+// inside vector (below the node) of internal node v at site k
+__device__ __forceinline__ void inside_node(const DevForest& F, int v, int k, const int32_t* __restrict__ done) {
   const int t = F.tree_of[v];
   const int p = F.tree_prob[t];
   if (k >= F.tree_nsites[t]) return;
@@ -99,15 +101,38 @@ __global__ void __launch_bounds__(256) k_inside(DevForest F, int beg, int cnt, c
   const int qo = F.site_qoff[sb + k] - 1;
   const double qa = la > 0 ? F.prior[qo + la] : 1.0;
   const double qb = lb > 0 ? F.prior[qo + lb] : 1.0;
-  SD e0a, eAa, e0b, eAb;
-  edge_eps(la, a0, aA, Ea, qa, cf1, e0a, eAa);
-  edge_eps(lb, b0, bA, Eb, qb, cf2, e0b, eAb);
-  const SD B0 = sd_mul(e0a, e0b);
-  const SD BA = lv != 0 ? sd_mul(eAa, eAb) : sd_zero();
-  int E = max(sd_top(B0), sd_top(BA));
-  if (E == kZeroTop) E = 0;
-  F.beta[rv + k] = make_double2(ldexp(B0.m, B0.e - E), ldexp(BA.m, BA.e - E));
-  F.bE[rv + k] = E;
+  const Eps ea = edge(la, a0, aA, Ea, qa, cf1);
+  const Eps eb = edge(lb, b0, bA, Eb, qb, cf2);
+  const double B0 = ea.e0 * eb.e0;
+  const double BA = lv != 0 ? ea.eA * eb.eA : 0.0;
+  // renormalise the pair by a power of two (bounded so that a denormal pair
+  // cannot produce an infinite scale)
+  const int sh = min(-exp_of(fmax(B0, BA)), 1000);
+  const double sc = ldexp(1.0, sh);
+  F.beta[rv + k] = make_double2(B0 * sc, BA * sc);
+  F.bE[rv + k] = ea.e + eb.e - sh;
+}
+
+template <int W>
+__global__ void __launch_bounds__(256) k_inside(DevForest F, int beg, int cnt, const int32_t* __restrict__ done) {
+  Seg<W> g;
+  if (!g.init(beg, cnt)) return;
+  inside_node(F, g.v, g.k, done);
+}
+
+// Inside pass over the top levels [h0, h1): one block per site chunk walks
+// the levels in order (its segments share each level's nodes), so the small
+// upper levels cost one launch instead of one each.
+template <int W>
+__global__ void __launch_bounds__(1024) k_inside_top(DevForest F, int h0, int h1, const int32_t* __restrict__ done) {
+  const int segs = 1024 / W;
+  const int seg = threadIdx.x / W;
+  const int k = (int)blockIdx.x * W + (int)(threadIdx.x & (W - 1));
+  for (int h = h0; h < h1; ++h) {
+    const int e = F.level_off[h + 1];
+    for (int v = F.level_off[h] + seg; v < e; v += segs) inside_node(F, v, k, done);
+    __syncthreads();
+  }
 }
 
 // Root of every tree: site likelihood L = P(D | stem starts unedited), the
@@ -137,33 +162,35 @@ __global__ void __launch_bounds__(256) k_root(DevForest F, double* __restrict__ 
     load_child(F, leaf, rr, k, phi, lab, b0, bA, E, code);
     const int qo = F.site_qoff[sb + k] - 1;
     const double q = lab > 0 ? F.prior[qo + lab] : 1.0;
-    SD e0, eA;
-    edge_eps(lab, b0, bA, E, q, cf, e0, eA);
-    int LE = sd_top(e0);
-    if (LE == kZeroTop) LE = 0;
-    const SD L = sd(ldexp(e0.m, e0.e - LE), LE);
-    F.Lm[sb + k] = L.m;
-    F.LE[sb + k] = L.e;
-    const double sl = sd_log(L);
+    const Eps er = edge(lab, b0, bA, E, q, cf);
+    const int ex = exp_of(er.e0);
+    const double Lm = ldexp(er.e0, -ex);
+    const int LE = er.e + ex;
+    F.Lm[sb + k] = Lm;
+    F.LE[sb + k] = LE;
+    const double sl = log(Lm) + (double)LE * 0.69314718055994530942;
     F.site_llh[sb + k] = sl;
     if (site_llh_out) site_llh_out[sb + k] = sl;
     if (!leaf) {
-      F.alpha[rr + k] = make_double4(cf.p1n, cf.sil, cf.mut, 0.0);
+      // outside state at the root: the stem starts unedited
+      F.alpha[rr + k] = make_double4(cf.p1n, ldexp(cf.msil, cf.esil), cf.mut, 0.0);
       F.aE[rr + k] = 0;
       F.aG[rr + k] = 0;
     }
     if (lab == 0) {
       acc0 = 1.0;
     } else {
-      acc0 = sd_div(sd(cf.p1n * b0, E), L);
-      acc1 = sd_div(sd(cf.mut * q * bA, E), L);
-      if (lab < 0) acc2 = sd_div(sd(cf.sil, 0), L);
+      const double iL = 1.0 / Lm;
+      const double f1 = ldexp(iL, min(E - LE, 1000));
+      acc0 = cf.p1n * b0 * f1;
+      acc1 = cf.mut * q * bA * f1;
+      if (lab < 0) acc2 = ldexp(iL * cf.msil, min(cf.esil - LE, 1000));
     }
   }
-  acc0 = seg_sum<W>(acc0);
-  acc1 = seg_sum<W>(acc1);
-  acc2 = seg_sum<W>(acc2);
-  if (lane == 0) {
+  acc0 = seg_sum_last<W>(acc0);
+  acc1 = seg_sum_last<W>(acc1);
+  acc2 = seg_sum_last<W>(acc2);
+  if (lane == W - 1) {
     double* c = F.cpart + ((int64_t)r * F.max_chunks + blockIdx.y) * 5;
     c[0] = acc0;
     c[1] = acc1;
@@ -177,15 +204,15 @@ __global__ void __launch_bounds__(256) k_root(DevForest F, double* __restrict__ 
 // posterior expected transition counts of both child branches and the
 // dropout responsibilities of missing leaf children (one partial per site
 // chunk).
+// All lanes of the W-lane segment call this together (segment shuffles).
 template <int W>
-__global__ void __launch_bounds__(256) k_outside(DevForest F, int beg, int cnt, const int32_t* __restrict__ done) {
-  Seg<W> g;
-  if (!g.init(beg, cnt)) return;
-  const int u = g.v, k = g.k, lane = g.lane;
+__device__ __forceinline__ void outside_node(const DevForest& F, int u, int chunk, int lane,
+                                             const int32_t* __restrict__ done) {
+  const int k = chunk * W + lane;
   const int t = F.tree_of[u];
   const int p = F.tree_prob[t];
   const int K = F.tree_nsites[t];
-  if ((int)blockIdx.y * W >= K) return;
+  if (chunk * W >= K) return;
   if (done && done[p]) return;
   const bool act = k < K;
   const int ch[2] = {F.child0[u], F.child1[u]};
@@ -205,74 +232,68 @@ __global__ void __launch_bounds__(256) k_outside(DevForest F, int beg, int cnt, 
     const double4 au = F.alpha[ru + k];
     const int Eo = F.aE[ru + k];
     const int gu = F.aG[ru + k];
-    // 1/L as a scaled value: posteriors are products with it
-    const SD iL = sd(1.0 / F.Lm[sb + k], -F.LE[sb + k]);
+    const double iL = 1.0 / F.Lm[sb + k];
+    const int LE = F.LE[sb + k];
     const int qo = F.site_qoff[sb + k] - 1;
     int lab[2], E[2], code[2];
     double b0[2], bA[2], q[2];
-    SD e0[2], eA[2];
+    Eps ep[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       load_child(F, lf[i], rc[i], k, phi, lab[i], b0[i], bA[i], E[i], code[i]);
       q[i] = lab[i] > 0 ? F.prior[qo + lab[i]] : 1.0;
-      edge_eps(lab[i], b0[i], bA[i], E[i], q[i], cf[i], e0[i], eA[i]);
+      ep[i] = edge(lab[i], b0[i], bA[i], E[i], q[i], cf[i]);
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int w = 1 - i;
-      // A = alpha_u(0) * eps_w(0): joint of the data outside v with u = 0
-      const SD A = sd(au.x * e0[w].m, Eo + e0[w].e);
-      // G(a) = alpha_u(a) eps_w(a) = q_a Gy + [a = gg] Gz ; GS = alpha_u(-1) eps_w(-1)
-      SD Gy = sd_zero(), Gz = sd_zero(), GS = sd_zero();
-      int gg = 0;
-      if (lab[w] < 0) {
-        Gy = sd(au.z * eA[w].m, Eo + eA[w].e);
-        Gz = sd(au.w * eA[w].m, Eo + eA[w].e);
-        gg = gu;
-        GS = sd(au.y, Eo);
-      } else if (lab[w] > 0) {
-        const double tq = fma(q[w], au.z, lab[w] == gu ? au.w : 0.0);
-        Gz = sd(tq * eA[w].m, Eo + eA[w].e);
-        gg = lab[w];
-      }
-      const SD sumG = sd_add(Gy, Gz);
-      const SD av0 = sd_scale(A, cf[i].p1n);
-      const SD base = sd_scale(A, cf[i].mut);
-      const SD Yv = sd_add(base, sd_scale(Gy, cf[i].pn));
-      const SD Zv = sd_scale(Gz, cf[i].pn);
+      const int lw = lab[w];
+      // values below share the exponent ea = Eo + e_w:
+      // A = alpha_u(0) eps_w(0) (joint of the data outside v with u = 0);
+      // G(a) = alpha_u(a) eps_w(a) = q_a Gy + [a = gg] Gz
+      const double A = au.x * ep[w].e0;
+      const double Gy = lw < 0 ? au.z * ep[w].eA : 0.0;
+      const double Gz = lw < 0 ? au.w * ep[w].eA
+                               : (lw > 0 ? fma(q[w], au.z, lw == gu ? au.w : 0.0) * ep[w].eA : 0.0);
+      const int gg = lw < 0 ? gu : (lw > 0 ? lw : 0);
+      const int ea = Eo + ep[w].e;
+      const double sumG = Gy + Gz;
+      const double av0 = A * cf[i].p1n;
+      const double base = A * cf[i].mut;
+      const double Yv = fma(cf[i].pn, Gy, base);
+      const double Zv = cf[i].pn * Gz;
       if (!lf[i]) {
-        const SD avS = sd_add(GS, sd_scale(sd_add(A, sumG), cf[i].sil));
-        int Ev = max(max(sd_top(av0), sd_top(avS)), max(sd_top(Yv), sd_top(Zv)));
-        if (Ev == kZeroTop) Ev = 0;
-        F.alpha[rc[i] + k] = make_double4(ldexp(av0.m, av0.e - Ev), ldexp(avS.m, avS.e - Ev),
-                                          ldexp(Yv.m, Yv.e - Ev), ldexp(Zv.m, Zv.e - Ev));
-        F.aE[rc[i] + k] = Ev;
+        // alpha_v(-1) = GS 2^Eo + sil (A + sumG) 2^ea, GS = alpha_u(-1) eps_w(-1).
+        // The stored exponent comes from the largest value of either term
+        // group (GS is often zero), so the quad is normalised in one step.
+        const double GS = lw < 0 ? au.y : 0.0;
+        const double T = ldexp(cf[i].msil * (A + sumG), cf[i].esil);
+        const int ta = exp_of(fmax(fmax(av0, T), fmax(Yv, Zv)));
+        const int tg = GS != 0.0 ? Eo + exp_of(GS) : kZeroTop;
+        const bool anya = fmax(fmax(av0, T), fmax(Yv, Zv)) != 0.0;
+        int ec = max(anya ? ea + ta : kZeroTop, tg);
+        if (ec == kZeroTop) ec = Eo;
+        const double sa = ldexp(1.0, min(ea - ec, 1000));
+        const double sg = ldexp(1.0, min(Eo - ec, 1000));
+        F.alpha[rc[i] + k] = make_double4(av0 * sa, fma(GS, sg, T * sa), Yv * sa, Zv * sa);
+        F.aE[rc[i] + k] = ec;
         F.aG[rc[i] + k] = (int16_t)gg;
       }
       const int lv = lab[i];
       if (lv == 0) {
         acc[i][0] = 1.0;
       } else {
-        // posterior terms share the factor 2^(E_v) / L; mantissas are
-        // multiplied first and the exponent applied once (values are <= 1)
-        const double ms = iL.m;
-        const int es = iL.e + E[i];
-        acc[i][0] = ldexp(av0.m * ms * b0[i], av0.e + es);
-        acc[i][1] = ldexp(base.m * ms * (q[i] * bA[i]), base.e + es);
-        SD t3;
-        if (lv > 0) {
-          t3 = sd_add(sd_scale(Gy, q[i]), lv == gg ? Gz : sd_zero());
-        } else {
-          t3 = sumG;
-        }
-        acc[i][3] = ldexp(t3.m * ms * (cf[i].pn * bA[i]), t3.e + es);
+        // S0, S1, S3 carry 2^(ea + E_v) / L; S2, S4 carry sil 2^ea / L
+        const double f1 = ldexp(iL, min(ea + E[i] - LE, 1000));
+        acc[i][0] = av0 * b0[i] * f1;
+        acc[i][1] = base * (q[i] * bA[i]) * f1;
+        const double t3 = lv > 0 ? fma(q[i], Gy, lv == gg ? Gz : 0.0) : sumG;
+        acc[i][3] = t3 * (cf[i].pn * bA[i]) * f1;
         if (lv < 0) {
-          acc[i][2] = ldexp(A.m * iL.m * cf[i].sil, A.e + iL.e);
-          acc[i][4] = ldexp(sumG.m * iL.m * cf[i].sil, sumG.e + iL.e);
-          if (lf[i] && code[i] == -1) {
-            const SD nsil = sd_add(av0, sd_add(Yv, Zv));
-            rt += ldexp(phi * nsil.m * iL.m, nsil.e + iL.e);
-          }
+          const double f2 = ldexp(iL * cf[i].msil, min(ea + cf[i].esil - LE, 1000));
+          acc[i][2] = A * f2;
+          acc[i][4] = sumG * f2;
+          if (lf[i] && code[i] == -1) rt += phi * (av0 + Yv + Zv) * ldexp(iL, min(ea - LE, 1000));
         }
       }
     }
@@ -280,18 +301,41 @@ __global__ void __launch_bounds__(256) k_outside(DevForest F, int beg, int cnt, 
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 5; ++j) acc[i][j] = seg_sum<W>(acc[i][j]);
-  rt = seg_sum<W>(rt);
-  if (lane == 0) {
+    for (int j = 0; j < 5; ++j) acc[i][j] = seg_sum_last<W>(acc[i][j]);
+  rt = seg_sum_last<W>(rt);
+  if (lane == W - 1) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      double* c = F.cpart + ((int64_t)ch[i] * F.max_chunks + blockIdx.y) * 5;
+      double* c = F.cpart + ((int64_t)ch[i] * F.max_chunks + chunk) * 5;
 #pragma unroll
       for (int j = 0; j < 5; ++j) c[j] = acc[i][j];
     }
-    F.rt_part[(int64_t)u * F.max_chunks + blockIdx.y] = rt;
+    F.rt_part[(int64_t)u * F.max_chunks + chunk] = rt;
   }
 }
+
+template <int W>
+__global__ void __launch_bounds__(256) k_outside(DevForest F, int beg, int cnt, const int32_t* __restrict__ done) {
+  Seg<W> g;
+  if (!g.init(beg, cnt)) return;
+  outside_node<W>(F, g.v, (int)blockIdx.y, g.lane, done);
+}
+
+// Outside pass over the top levels, from level h1-1 down to h0, one block per
+// site chunk (see k_inside_top).
+template <int W>
+__global__ void __launch_bounds__(1024) k_outside_top(DevForest F, int h0, int h1, const int32_t* __restrict__ done) {
+  const int segs = 1024 / W;
+  const int seg = threadIdx.x / W;
+  const int lane = threadIdx.x & (W - 1);
+  for (int h = h1 - 1; h >= h0; --h) {
+    const int e = F.level_off[h + 1];
+    for (int u = F.level_off[h] + seg; u < e; u += segs) outside_node<W>(F, u, (int)blockIdx.x, lane, done);
+    __syncthreads();
+  }
+}
+
+
 
 // Per-node counts: fixed-order sum of the per-chunk partials.
 __global__ void k_counts(DevForest F, double* __restrict__ counts) {
@@ -638,12 +682,20 @@ void run_coef(const DevForest& F, const double* d, const double* nu, const int32
   hipLaunchKernelGGL(k_coef, dim3(grid), dim3(256), 0, s, F, d, nu, done);
 }
 
-void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, const int32_t* done, hipStream_t s) {
-  for (size_t h = 1; h + 1 < lvl.size(); ++h) {
+void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const int32_t* done,
+                hipStream_t s) {
+  const int H = (int)lvl.size() - 1;
+  const int ht = std::max(1, std::min(h_top, H));
+  for (int h = 1; h < ht; ++h) {
     const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
     if (cnt <= 0) continue;
     LAML_DISPATCH_W(W, {
       hipLaunchKernelGGL(k_inside<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt, done);
+    });
+  }
+  if (ht < H) {
+    LAML_DISPATCH_W(W, {
+      hipLaunchKernelGGL(k_inside_top<WW>, dim3(F.max_chunks), dim3(1024), 0, s, F, ht, H, done);
     });
   }
 }
@@ -655,8 +707,16 @@ void run_root(const DevForest& F, int W, double* site_llh_out, const int32_t* do
   });
 }
 
-void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, const int32_t* done, hipStream_t s) {
-  for (size_t h = lvl.size() - 2; h >= 1; --h) {
+void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const int32_t* done,
+                 hipStream_t s) {
+  const int H = (int)lvl.size() - 1;
+  const int ht = std::max(1, std::min(h_top, H));
+  if (ht < H) {
+    LAML_DISPATCH_W(W, {
+      hipLaunchKernelGGL(k_outside_top<WW>, dim3(F.max_chunks), dim3(1024), 0, s, F, ht, H, done);
+    });
+  }
+  for (int h = ht - 1; h >= 1; --h) {
     const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
     if (cnt <= 0) continue;
     LAML_DISPATCH_W(W, {

@@ -23,6 +23,7 @@ struct DevForest {
   int32_t max_chunks;  // ceil(max K / W)
   double dmin, dmax;
   // topology (level order)
+  const int32_t* level_off;  // [levels+1]
   const int32_t* child0;
   const int32_t* child1;
   const int32_t* tree_of;
@@ -80,9 +81,11 @@ struct EmState {
 int seg_width(int max_k);
 void run_labels(const DevForest& F, const std::vector<int32_t>& lvl, int W, hipStream_t s);
 void run_coef(const DevForest& F, const double* d, const double* nu, const int32_t* done, hipStream_t s);
-void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, const int32_t* done, hipStream_t s);
+void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const int32_t* done,
+                hipStream_t s);
 void run_root(const DevForest& F, int W, double* site_llh_out, const int32_t* done, hipStream_t s);
-void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, const int32_t* done, hipStream_t s);
+void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const int32_t* done,
+                 hipStream_t s);
 void run_counts(const DevForest& F, double* counts, hipStream_t s);
 void run_llh(const DevForest& F, double* nll, EmState em, int step, int max_iter, double tol, hipStream_t s);
 void run_rtilde(const DevForest& F, double* rtilde, const int32_t* done, hipStream_t s);

@@ -193,3 +193,46 @@ def test_batched_forest_equals_single():
     phi = torch.full((len(specs),), 0.1, dtype=torch.float64, device=F.device)
     got = F.nll(d, nu, phi).cpu().numpy()
     np.testing.assert_allclose(got, singles, rtol=NLL_RTOL)
+
+
+@pytest.mark.parametrize("n,K,seed", [(10000, 12, 20261206), (3000, 24, 11)])
+def test_deep_tree_nll_and_counts(n, K, seed):
+    """Deep Yule trees (height ~30): exponent ranges far beyond the small
+    fixtures.  Likelihood and one E-step against the oracle."""
+    tree, Q, codes, _ = sim.make_problem(n, K, 5, 60, 1.0, 0.1, 0.1, seed)
+    T = tree.newick()
+    msa = sim.codes_to_charmtrx(tree, codes)
+    o = OracleEM([T], msa, Q, nu=0.1, phi=0.1)
+    s = EM_solver([T], {"charMtrx": msa}, {"Q": Q}, {"nu": 0.1, "phi": 0.1})
+    want = o.negative_llh()
+    got = s.negative_llh()
+    print("deep", n, K, got, want, flush=True)
+    assert math.isclose(got, want, rel_tol=NLL_RTOL)
+    o.estep()
+    s.Estep()
+    worst = 0.0
+    for vo, vs in zip(postorder(o.trees[0]), _post_nodes(s)):
+        want_c = np.array([sum(vo.a["S"][j]) for j in range(5)])
+        worst = max(worst, float(np.max(np.abs(np.array(vs.S_sum) - want_c))))
+    print("deep counts max abs diff", worst, flush=True)
+    assert worst < 1e-8 * K
+
+
+def test_deep_tree_em_from_uniform_start():
+    """The benchmark's start (all branches 0.05, nu = phi = 0.5) on a deep
+    10k-leaf tree with few sites: EM trajectory against the oracle."""
+    tree, Q, codes, _ = sim.make_problem(10000, 8, 5, 60, 1.0, 0.1, 0.1, 20261206)
+    T = tree.newick()
+    msa = sim.codes_to_charmtrx(tree, codes)
+    o, s = _pair(T, msa, Q, 0.5, 0.5)
+    _uniform_start(s, o)
+    iters = 4
+    o.az_partition()
+    onll, _, _ = o.em_optimization(fixed_iters=iters)
+    F = s._forest_for()
+    d, nu_t, phi_t = s._device_params(F)
+    trace = torch.zeros(iters + 1, dtype=torch.float64, device=F.device)
+    nll, it = F.em(d, nu_t, phi_t, iters, 0.0, trace=trace)
+    print("deep em", (-trace).cpu().numpy(), o.llh_trace, flush=True)
+    np.testing.assert_allclose((-trace).cpu().numpy(), o.llh_trace, rtol=NLL_RTOL)
+    assert abs(nu_t.item() - o.nu) < 1e-6 and abs(phi_t.item() - o.phi) < 1e-6
