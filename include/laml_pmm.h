@@ -147,6 +147,40 @@ int laml_mstep(laml_forest* f, const double* counts, const double* rtilde,
 int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter, double tol,
             double* nll, int32_t* iters, double* trace, void* stream);
 
+/* Height-class model of the ultrametric constraint (see laml_amd/ultra.py).
+ * Classes are sorted by (class level, problem); level_problem_off[l*P + p]
+ * is the first class of problem p at class level l. */
+typedef struct {
+  int32_t num_classes;
+  int32_t num_levels;
+  const int32_t* level_problem_off; /* [num_levels * num_problems + 1]        */
+  const int32_t* up_cls;            /* [num_nodes] parent-side class, -1 pinned */
+  const int32_t* dn_cls;            /* [num_nodes] child-side class, -1 pinned  */
+  const double* up_off;             /* [num_nodes] offset / absolute height    */
+  const double* dn_off;             /* [num_nodes]                             */
+  const int32_t* cls_parent;        /* [num_classes] -1 at the top             */
+  const int32_t* cls_pedge;         /* [num_classes] branch to the parent class */
+  const int32_t* inc_off;           /* [num_classes + 1]                       */
+  const int32_t* inc_branch;        /* incident free branches                  */
+  const int8_t* inc_sign;           /* +1 where the class is the parent side   */
+  const uint8_t* problem_has_free;  /* [num_problems]                          */
+} laml_ultra_desc;
+
+/* Switch the M-step of every problem to the ultrametric-constrained branch
+ * problem (NULL switches back to the unconstrained one).
+ * Replaces: EM_solver.ultrametric_constr + the constrained MOSEK solve
+ * (EM_solver.py:53-89, 349-366, used when ultra_constr=True). */
+int laml_forest_set_ultrametric(laml_forest* f, const laml_ultra_desc* u);
+
+/* Starting heights of the variable classes (host [num_classes]); they must
+ * give every free branch a length strictly inside (dmin, dmax).  Later
+ * ultrametric M-steps warm-start from the previous solution. */
+int laml_forest_set_heights(laml_forest* f, const double* heights_host);
+
+/* Per-problem status of the last ultrametric M-step (0 ok, 1 line search
+ * stalled), host int32 [num_problems]. */
+int laml_forest_ultra_status(laml_forest* f, int32_t* status_host);
+
 /* Text of the last error on this thread ("" if none). */
 const char* laml_last_error(void);
 

@@ -51,6 +51,27 @@ class ForestDesc(ctypes.Structure):
     ]
 
 
+_i8p = ctypes.POINTER(ctypes.c_int8)
+
+
+class UltraDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_classes", ctypes.c_int32),
+        ("num_levels", ctypes.c_int32),
+        ("level_problem_off", _i32p),
+        ("up_cls", _i32p),
+        ("dn_cls", _i32p),
+        ("up_off", _f64p),
+        ("dn_off", _f64p),
+        ("cls_parent", _i32p),
+        ("cls_pedge", _i32p),
+        ("inc_off", _i32p),
+        ("inc_branch", _i32p),
+        ("inc_sign", _i8p),
+        ("problem_has_free", _u8p),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/laml_pmm.h declares
 SIGNATURES = {
     "laml_forest_create": (ctypes.c_int, [ctypes.POINTER(ForestDesc), ctypes.c_int,
@@ -66,6 +87,9 @@ SIGNATURES = {
     "laml_em": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                ctypes.c_int32, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
                                ctypes.c_void_p, ctypes.c_void_p]),
+    "laml_forest_set_ultrametric": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(UltraDesc)]),
+    "laml_forest_set_heights": (ctypes.c_int, [ctypes.c_void_p, _f64p]),
+    "laml_forest_ultra_status": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
     "laml_last_error": (ctypes.c_char_p, []),
     "laml_version": (ctypes.c_char_p, []),
 }

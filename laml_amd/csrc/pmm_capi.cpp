@@ -46,6 +46,37 @@ struct laml_forest {
   double* phi_buf = nullptr; // [P] staging when the caller passes phi
   EmState em{};
   int32_t* em_done = nullptr;
+  // ultrametric model (owned separately so it can be replaced)
+  bool ultra_on = false;
+  laml::UltraDev U{};
+  std::vector<void*> ultra_allocs;
+  std::vector<int32_t> ultra_status_h;
+
+  void free_ultra() {
+    for (void* p : ultra_allocs) (void)hipFree(p);
+    ultra_allocs.clear();
+    ultra_on = false;
+    U = laml::UltraDev{};
+  }
+  template <typename T>
+  int ualloc(T** p, size_t n) {
+    void* q = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) return fail(LAML_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    ultra_allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return LAML_OK;
+  }
+  template <typename T>
+  int uupload(const T** dst, const T* src, size_t n) {
+    T* p = nullptr;
+    int rc = ualloc(&p, n);
+    if (rc) return rc;
+    if (n) LAML_HIP(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    *dst = p;
+    return LAML_OK;
+  }
 
   template <typename T>
   int alloc(T** p, size_t n) {
@@ -68,6 +99,7 @@ struct laml_forest {
   }
   ~laml_forest() {
     for (void* p : allocs) (void)hipFree(p);
+    for (void* p : ultra_allocs) (void)hipFree(p);
   }
 };
 
@@ -404,6 +436,91 @@ int laml_forest_labels(laml_forest* f, int16_t* labels_host, int64_t capacity) {
   return LAML_OK;
 }
 
+int laml_forest_set_ultrametric(laml_forest* f, const laml_ultra_desc* u) {
+  if (!f) return fail(LAML_EINVAL, "null forest");
+  DeviceGuard guard(f->device);
+  LAML_HIP(hipDeviceSynchronize());
+  f->free_ultra();
+  if (!u) return LAML_OK;
+  const int N = f->F.num_nodes, P = f->F.num_problems, V = u->num_classes, H = u->num_levels;
+  if (V < 0 || H < 0 || (V > 0 && H <= 0)) return fail(LAML_EINVAL, "bad class counts");
+  if (!u->level_problem_off || !u->up_cls || !u->dn_cls || !u->up_off || !u->dn_off || !u->cls_parent ||
+      !u->cls_pedge || !u->inc_off || !u->inc_branch || !u->inc_sign || !u->problem_has_free)
+    return fail(LAML_EINVAL, "null array in ultrametric description");
+  const int64_t LP = (int64_t)H * P + 1;
+  if (u->level_problem_off[0] != 0 || u->level_problem_off[LP - 1] != V)
+    return fail(LAML_EINVAL, "level_problem_off must span the classes");
+  for (int64_t i = 0; i + 1 < LP; ++i)
+    if (u->level_problem_off[i + 1] < u->level_problem_off[i]) return fail(LAML_EINVAL, "level_problem_off not monotone");
+  for (int v = 0; v < N; ++v) {
+    if (u->up_cls[v] < -1 || u->up_cls[v] >= V || u->dn_cls[v] < -1 || u->dn_cls[v] >= V)
+      return fail(LAML_EINVAL, "branch class out of range");
+  }
+  if (u->inc_off[0] != 0) return fail(LAML_EINVAL, "inc_off[0] != 0");
+  for (int c = 0; c < V; ++c) {
+    if (u->inc_off[c + 1] <= u->inc_off[c]) return fail(LAML_EINVAL, "class without incident free branch");
+    if (u->cls_parent[c] < -1 || u->cls_parent[c] >= V) return fail(LAML_EINVAL, "cls_parent out of range");
+    if (u->cls_parent[c] >= 0 && (u->cls_pedge[c] < 0 || u->cls_pedge[c] >= N))
+      return fail(LAML_EINVAL, "cls_pedge out of range");
+  }
+  const int NI = u->inc_off[V];
+  for (int j = 0; j < NI; ++j)
+    if (u->inc_branch[j] < 0 || u->inc_branch[j] >= N) return fail(LAML_EINVAL, "inc_branch out of range");
+  laml::UltraDev& U = f->U;
+  U.num_classes = V;
+  U.num_levels = H;
+  int rc;
+#define UU(dst, src, n) \
+  if ((rc = f->uupload(&U.dst, src, (size_t)(n)))) { f->free_ultra(); return rc; }
+  UU(plo, u->level_problem_off, LP);
+  UU(up_cls, u->up_cls, N);
+  UU(dn_cls, u->dn_cls, N);
+  UU(up_off, u->up_off, N);
+  UU(dn_off, u->dn_off, N);
+  UU(cls_parent, u->cls_parent, V);
+  UU(cls_pedge, u->cls_pedge, V);
+  UU(inc_off, u->inc_off, V + 1);
+  UU(inc_branch, u->inc_branch, NI);
+  UU(inc_sign, u->inc_sign, NI);
+  UU(has_free, u->problem_has_free, P);
+#undef UU
+#define UA(dst, type, n) \
+  if ((rc = f->ualloc<type>(&U.dst, (size_t)(n)))) { f->free_ultra(); return rc; }
+  UA(t, double, V);
+  UA(dt, double, V);
+  UA(Dd, double, V);
+  UA(r, double, V);
+  UA(g, double, V);
+  UA(gp, double, N);
+  UA(w, double, N);
+  UA(bA, double, N);
+  UA(bS1, double, N);
+  UA(bB, double, N);
+  UA(status, int32_t, P);
+#undef UA
+  LAML_HIP(hipMemset(U.status, 0, sizeof(int32_t) * P));
+  LAML_HIP(hipMemset(U.t, 0, sizeof(double) * (V > 0 ? V : 1)));
+  f->ultra_on = true;
+  return LAML_OK;
+}
+
+int laml_forest_set_heights(laml_forest* f, const double* heights_host) {
+  if (!f || !heights_host) return fail(LAML_EINVAL, "null argument");
+  if (!f->ultra_on) return fail(LAML_ESTATE, "no ultrametric model set");
+  DeviceGuard guard(f->device);
+  if (f->U.num_classes > 0)
+    LAML_HIP(hipMemcpy(f->U.t, heights_host, sizeof(double) * f->U.num_classes, hipMemcpyHostToDevice));
+  return LAML_OK;
+}
+
+int laml_forest_ultra_status(laml_forest* f, int32_t* status_host) {
+  if (!f || !status_host) return fail(LAML_EINVAL, "null argument");
+  if (!f->ultra_on) return fail(LAML_ESTATE, "no ultrametric model set");
+  DeviceGuard guard(f->device);
+  LAML_HIP(hipMemcpy(status_host, f->U.status, sizeof(int32_t) * f->F.num_problems, hipMemcpyDeviceToHost));
+  return LAML_OK;
+}
+
 static int check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(LAML_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -461,7 +578,7 @@ int laml_mstep(laml_forest* f, const double* counts, const double* rtilde, doubl
   if (!f || !counts || !rtilde || !d || !nu || !phi) return fail(LAML_EINVAL, "null argument");
   DeviceGuard guard(f->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  laml::run_mstep(f->F, counts, rtilde, d, nu, phi, nullptr, s);
+  laml::run_mstep(f->F, counts, rtilde, d, nu, phi, nullptr, s, f->ultra_on ? &f->U : nullptr);
   return check_launch();
 }
 
@@ -487,7 +604,7 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
     laml::run_llh(F, nll, em, step, max_iter, tol, s);
     if (step < max_iter) {
       laml::run_outside(F, f->level_off, f->W, f->h_top, em.done, s);
-      laml::run_mstep(F, nullptr, nullptr, d, nu, phi, em.done, s);
+      laml::run_mstep(F, nullptr, nullptr, d, nu, phi, em.done, s, f->ultra_on ? &f->U : nullptr);
     }
     int rc = check_launch();
     if (rc) return rc;

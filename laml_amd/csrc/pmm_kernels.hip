@@ -473,7 +473,7 @@ __device__ double solve_branch(const BranchObj& o, double dmin, double dmax, dou
 // problem's per-branch coefficients at the branch's position in its problem.
 __global__ void __launch_bounds__(64) k_branch(DevForest F, const double* __restrict__ counts,
                                                 double* __restrict__ d, const double* __restrict__ nu,
-                                                const int32_t* __restrict__ done) {
+                                                const int32_t* __restrict__ done, UltraDev U, int ultra) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= F.num_nodes) return;
   const int t = F.tree_of[v];
@@ -514,6 +514,16 @@ __global__ void __launch_bounds__(64) k_branch(DevForest F, const double* __rest
   o.A = (nu0 + 1.0) * s[0] + nu0 * s[1] + nu0 * s[3];
   o.S1 = s[1];
   o.B = nu0 > kEpsNu ? (s[2] + s[4]) : 0.0;
+  if (ultra) {
+    // the ultrametric solve needs the objective of every free branch; the
+    // lengths are set by k_ultra and the nu staging refreshed after it
+    U.bA[v] = o.A;
+    U.bS1[v] = o.S1;
+    U.bB[v] = o.B;
+    F.nu_a[pos] = s[0] + s[1] + s[3];
+    F.nu_b[pos] = s[2] + s[4];
+    return;
+  }
   const double dn = solve_branch(o, F.dmin, F.dmax, d[v]);
   d[v] = dn;
   F.nu_a[pos] = s[0] + s[1] + s[3];
@@ -645,6 +655,195 @@ __global__ void __launch_bounds__(kNuThreads) k_nu_eval(DevForest F, const doubl
 }
 
 // ---------------------------------------------------------------------------
+// Ultrametric branch-length M-step (EM_solver.py:349-366 with the equality
+// system of EM_solver.py:53-89).  Heights of the variable height classes are
+// the unknowns, d_v = t(parent side) - t(child side).  A log-barrier keeps
+// every free branch in (dmin, dmax); each Newton system is a weighted tree
+// Laplacian over the classes and is solved exactly by eliminating classes
+// bottom-up and substituting top-down (two level sweeps).  One block per
+// problem; every decision depends on block-reduced values, so control flow is
+// uniform across the block.
+// ---------------------------------------------------------------------------
+constexpr int kUltraThreads = 1024;
+
+template <int NT>
+__device__ __forceinline__ double block_min(double x, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = fmin(x, __shfl_xor(x, off, 64));
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[wid] = x;
+  __syncthreads();
+  double s = sh[0];
+#pragma unroll
+  for (int i = 1; i < NT / 64; ++i) s = fmin(s, sh[i]);
+  __syncthreads();
+  return s;
+}
+
+__device__ __forceinline__ double ult_height(const UltraDev& U, int c, double off, double a) {
+  return c >= 0 ? U.t[c] + a * U.dt[c] + off : off;
+}
+
+// branch objective (barrier excluded): value, first and second derivative
+__device__ __forceinline__ void ult_obj(double A, double S1, double B, double nu, double d, double& f, double& g,
+                                        double& h) {
+  const double em = expm1(d);
+  f = -A * d + S1 * log(-expm1(-d));
+  g = -A + S1 / em;
+  h = -S1 * (em + 1.0) / (em * em);
+  if (B > 0.0) {
+    const double en = expm1(nu * d);
+    f += B * log(-expm1(-nu * d));
+    g += B * nu / en;
+    h -= B * nu * nu * (en + 1.0) / (en * en);
+  }
+}
+
+// barrier objective of problem p's free branches at heights t + a dt
+__device__ double ult_phi(const DevForest& F, const UltraDev& U, int nb, int ne, double nu0, double mu, double a,
+                          double* sh) {
+  double s = 0.0;
+  for (int i = nb + threadIdx.x; i < ne; i += kUltraThreads) {
+    const int v = F.prob_nodes[i];
+    if (!F.free_branch[v]) continue;
+    const double dv = ult_height(U, U.up_cls[v], U.up_off[v], a) - ult_height(U, U.dn_cls[v], U.dn_off[v], a);
+    const double lo = dv - F.dmin, hi = F.dmax - dv;
+    if (!(lo > 0.0 && hi > 0.0)) {
+      s = -INFINITY;
+      continue;
+    }
+    double f, g, h;
+    ult_obj(U.bA[v], U.bS1[v], U.bB[v], nu0, dv, f, g, h);
+    s += f + mu * (log(lo) + log(hi));
+  }
+  return block_sum<kUltraThreads>(s, sh);
+}
+
+__global__ void __launch_bounds__(kUltraThreads) k_ultra(DevForest F, UltraDev U, double* __restrict__ d,
+                                                         const double* __restrict__ nu,
+                                                         const int32_t* __restrict__ done) {
+  __shared__ double sh[kUltraThreads / 64];
+  const int p = blockIdx.x;
+  if ((done && done[p]) || !U.has_free[p]) return;
+  const int P = F.num_problems;
+  const int nb = F.prob_node_off[p], ne = F.prob_node_off[p + 1];
+  const double nu0 = nu[p];
+  const double dmin = F.dmin, dmax = F.dmax;
+  const double K = (double)F.tree_nsites[F.prob_tree_off[p]];
+  const int H = U.num_levels;
+  int status = 0;
+  double mu = 1e-2 * K;
+  const double mu_end = 1e-13 * K;
+  for (int outer = 0; outer < 40; ++outer) {
+    for (int it = 0; it < 60; ++it) {
+      // branch derivatives of the barrier objective
+      for (int i = nb + threadIdx.x; i < ne; i += kUltraThreads) {
+        const int v = F.prob_nodes[i];
+        if (!F.free_branch[v]) continue;
+        const double dv = ult_height(U, U.up_cls[v], U.up_off[v], 0.0) - ult_height(U, U.dn_cls[v], U.dn_off[v], 0.0);
+        const double lo = dv - dmin, hi = dmax - dv;
+        double f, g, h;
+        ult_obj(U.bA[v], U.bS1[v], U.bB[v], nu0, dv, f, g, h);
+        U.gp[v] = g + mu * (1.0 / lo - 1.0 / hi);
+        U.w[v] = -h + mu * (1.0 / (lo * lo) + 1.0 / (hi * hi));
+      }
+      __syncthreads();
+      // eliminate classes bottom-up
+      for (int lv = 0; lv < H; ++lv) {
+        const int cb = U.plo[lv * P + p], ce = U.plo[lv * P + p + 1];
+        for (int c = cb + threadIdx.x; c < ce; c += kUltraThreads) {
+          double gg = 0.0, L = 0.0, rr = 0.0;
+          for (int j = U.inc_off[c]; j < U.inc_off[c + 1]; ++j) {
+            const int v = U.inc_branch[j];
+            const double wv = U.w[v];
+            if (U.inc_sign[j] > 0) {
+              gg += U.gp[v];
+              const int kc = U.dn_cls[v];
+              if (kc >= 0) {
+                L -= wv * wv / U.Dd[kc];
+                rr += wv * U.r[kc] / U.Dd[kc];
+              }
+            } else {
+              gg -= U.gp[v];
+            }
+            L += wv;
+          }
+          U.g[c] = gg;
+          U.Dd[c] = L;
+          U.r[c] = gg + rr;
+        }
+        __syncthreads();
+      }
+      // substitute top-down
+      for (int lv = H - 1; lv >= 0; --lv) {
+        const int cb = U.plo[lv * P + p], ce = U.plo[lv * P + p + 1];
+        for (int c = cb + threadIdx.x; c < ce; c += kUltraThreads) {
+          const int pc = U.cls_parent[c];
+          U.dt[c] = pc < 0 ? U.r[c] / U.Dd[c] : (U.r[c] + U.w[U.cls_pedge[c]] * U.dt[pc]) / U.Dd[c];
+        }
+        __syncthreads();
+      }
+      // Newton decrement, largest step keeping every branch inside the box
+      double lam = 0.0, step = 0.0;
+      const int c0 = U.plo[p], c1 = U.plo[(H - 1) * P + p + 1];
+      for (int lv = 0; lv < H; ++lv) {
+        const int cb = U.plo[lv * P + p], ce = U.plo[lv * P + p + 1];
+        for (int c = cb + threadIdx.x; c < ce; c += kUltraThreads) {
+          lam += U.g[c] * U.dt[c];
+          step = fmax(step, fabs(U.dt[c]) / (1.0 + fabs(U.t[c])));
+        }
+      }
+      (void)c0;
+      (void)c1;
+      double amax = 1.0;
+      for (int i = nb + threadIdx.x; i < ne; i += kUltraThreads) {
+        const int v = F.prob_nodes[i];
+        if (!F.free_branch[v]) continue;
+        const double dv = ult_height(U, U.up_cls[v], U.up_off[v], 0.0) - ult_height(U, U.dn_cls[v], U.dn_off[v], 0.0);
+        const double dd = (U.up_cls[v] >= 0 ? U.dt[U.up_cls[v]] : 0.0) - (U.dn_cls[v] >= 0 ? U.dt[U.dn_cls[v]] : 0.0);
+        if (dd < 0.0) amax = fmin(amax, 0.99 * (dv - dmin) / (-dd));
+        else if (dd > 0.0) amax = fmin(amax, 0.99 * (dmax - dv) / dd);
+      }
+      lam = block_sum<kUltraThreads>(lam, sh);
+      step = -block_min<kUltraThreads>(-step, sh);
+      amax = block_min<kUltraThreads>(amax, sh);
+      if (!(lam > 0.0) || step * amax <= 1e-15) break;
+      double a = amax;
+      const double phi0 = ult_phi(F, U, nb, ne, nu0, mu, 0.0, sh);
+      if (lam > 1e-12 * (1.0 + fabs(phi0))) {
+        // backtracking (Armijo) while the predicted gain is measurable
+        int k = 0;
+        for (; k < 40; ++k) {
+          const double phi1 = ult_phi(F, U, nb, ne, nu0, mu, a, sh);
+          if (phi1 >= phi0 + 0.25 * a * lam) break;
+          a *= 0.5;
+        }
+        if (k == 40) {
+          status = 1;
+          break;
+        }
+      }
+      for (int lv = 0; lv < H; ++lv) {
+        const int cb = U.plo[lv * P + p], ce = U.plo[lv * P + p + 1];
+        for (int c = cb + threadIdx.x; c < ce; c += kUltraThreads) U.t[c] += a * U.dt[c];
+      }
+      __syncthreads();
+      if (step * a <= 1e-15) break;
+    }
+    if (mu <= mu_end) break;
+    mu = fmax(mu * 0.1, mu_end);
+  }
+  for (int i = nb + threadIdx.x; i < ne; i += kUltraThreads) {
+    const int v = F.prob_nodes[i];
+    if (!F.free_branch[v]) continue;
+    const double dv = ult_height(U, U.up_cls[v], U.up_off[v], 0.0) - ult_height(U, U.dn_cls[v], U.dn_off[v], 0.0);
+    d[v] = dv;
+    F.nu_d[F.node_pos[v]] = dv;
+  }
+  if (threadIdx.x == 0) U.status[p] = status;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 int seg_width(int max_k) {
@@ -738,8 +937,12 @@ void run_rtilde(const DevForest& F, double* rtilde, const int32_t* done, hipStre
 }
 
 void run_mstep(const DevForest& F, const double* counts, const double* rtilde, double* d, double* nu, double* phi,
-               const int32_t* done, hipStream_t s) {
-  hipLaunchKernelGGL(k_branch, dim3((F.num_nodes + 63) / 64), dim3(64), 0, s, F, counts, d, nu, done);
+               const int32_t* done, hipStream_t s, const UltraDev* U) {
+  UltraDev none{};
+  hipLaunchKernelGGL(k_branch, dim3((F.num_nodes + 63) / 64), dim3(64), 0, s, F, counts, d, nu, done,
+                     U ? *U : none, U ? 1 : 0);
+  if (U)
+    hipLaunchKernelGGL(k_ultra, dim3(F.num_problems), dim3(kUltraThreads), 0, s, F, *U, d, nu, done);
   for (int it = 0; it <= kNuIters; ++it)
     hipLaunchKernelGGL(k_nu_eval, dim3(F.num_nu_blocks), dim3(kNuThreads), 0, s, F, rtilde, nu, phi, done, it);
 }

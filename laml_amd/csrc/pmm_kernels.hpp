@@ -71,6 +71,33 @@ struct DevForest {
   NuState* nu_state;  // [P]
 };
 
+// Height-class model of the ultrametric constraint (one per forest)
+struct UltraDev {
+  int32_t num_classes, num_levels;
+  const int32_t* plo;        // [num_levels * P + 1] class ranges by (level, problem)
+  const int32_t* up_cls;     // [N] parent-side class of each free branch (-1 pinned)
+  const int32_t* dn_cls;     // [N] child-side class (-1 pinned)
+  const double* up_off;      // [N] offset (absolute height when pinned)
+  const double* dn_off;      // [N]
+  const int32_t* cls_parent; // [V]
+  const int32_t* cls_pedge;  // [V] branch joining the class to its parent class
+  const int32_t* inc_off;    // [V+1]
+  const int32_t* inc_branch; // incident free branches
+  const int8_t* inc_sign;    // +1: class is the parent side
+  const uint8_t* has_free;   // [P]
+  double* t;                 // [V] heights (warm start / solution)
+  double* dt;                // [V] Newton direction
+  double* Dd;                // [V] eliminated diagonal
+  double* r;                 // [V] eliminated right-hand side
+  double* g;                 // [V] gradient
+  double* gp;                // [N] branch first derivative
+  double* w;                 // [N] branch curvature (positive)
+  double* bA;                // [N] branch objective: -A d + S1 log(1-e^-d) + B log(1-e^-nu d)
+  double* bS1;
+  double* bB;
+  int32_t* status;           // [P] 0 ok, 1 line search stalled
+};
+
 struct EmState {
   int32_t* done;   // [P] or nullptr outside the EM loop
   int32_t* iters;  // [P]
@@ -90,6 +117,6 @@ void run_counts(const DevForest& F, double* counts, hipStream_t s);
 void run_llh(const DevForest& F, double* nll, EmState em, int step, int max_iter, double tol, hipStream_t s);
 void run_rtilde(const DevForest& F, double* rtilde, const int32_t* done, hipStream_t s);
 void run_mstep(const DevForest& F, const double* counts, const double* rtilde, double* d, double* nu, double* phi,
-               const int32_t* done, hipStream_t s);
+               const int32_t* done, hipStream_t s, const UltraDev* U = nullptr);
 
 }  // namespace laml

@@ -29,9 +29,9 @@ class TreeArrays:
     """Post-order arrays of one binary tree."""
 
     __slots__ = ("n_nodes", "n_leaves", "c0", "c1", "parent", "height", "length", "free",
-                 "leaf_post", "codes", "nodes")
+                 "leaf_post", "codes", "nodes", "poly")
 
-    def __init__(self, c0, c1, parent, length, free, codes, nodes=None):
+    def __init__(self, c0, c1, parent, length, free, codes, nodes=None, poly=None):
         self.c0 = np.asarray(c0, dtype=np.int64)
         self.c1 = np.asarray(c1, dtype=np.int64)
         self.parent = np.asarray(parent, dtype=np.int64)
@@ -43,6 +43,7 @@ class TreeArrays:
         self.n_leaves = len(self.leaf_post)
         self.codes = codes  # int16 [n_leaves, K] rows in leaf post-order
         self.nodes = nodes
+        self.poly = np.zeros(self.n_nodes, dtype=bool) if poly is None else np.asarray(poly, dtype=bool)
         h = np.zeros(self.n_nodes, dtype=np.int64)
         # post-order: children precede parents
         for v in range(self.n_nodes):
@@ -53,7 +54,7 @@ class TreeArrays:
         self.height = h
 
 
-def tree_arrays_from_tree(tree, codes_of_leaf, free_of_node=None):
+def tree_arrays_from_tree(tree, codes_of_leaf, free_of_node=None, poly_of_node=None):
     """Post-order arrays of a binary ``laml_amd.tree.Tree``; ``codes_of_leaf``
     maps a leaf Node to its int16 code row."""
     nodes = list(tree.traverse_postorder())
@@ -64,8 +65,11 @@ def tree_arrays_from_tree(tree, codes_of_leaf, free_of_node=None):
     parent = np.full(N, -1, dtype=np.int64)
     length = np.zeros(N, dtype=np.float64)
     free = np.ones(N, dtype=np.uint8)
+    poly = np.zeros(N, dtype=bool)
     rows = []
     for i, v in enumerate(nodes):
+        if poly_of_node is not None:
+            poly[i] = bool(poly_of_node(v))
         if v.children:
             if len(v.children) != 2:
                 raise ValueError("device forest requires binary trees (resolve polytomies first)")
@@ -81,7 +85,7 @@ def tree_arrays_from_tree(tree, codes_of_leaf, free_of_node=None):
     if N < 3:
         raise ValueError("device forest requires trees with at least two leaves")
     codes = np.stack(rows).astype(np.int16) if rows else np.zeros((0, 0), np.int16)
-    return TreeArrays(c0, c1, parent, length, free, codes, nodes)
+    return TreeArrays(c0, c1, parent, length, free, codes, nodes, poly)
 
 
 class SiteTable:
@@ -166,6 +170,7 @@ class ForestLayout:
         c1 = np.concatenate([gmap(t.c1, i) for i, t in enumerate(trees)])
         par = np.concatenate([gmap(t.parent, i) for i, t in enumerate(trees)])
         free = np.concatenate([t.free for t in trees])
+        poly = np.concatenate([t.poly for t in trees])
 
         def relabel(a):
             a = a[order]
@@ -176,6 +181,7 @@ class ForestLayout:
         self.parent = relabel(par)
         self.tree_of = tree_of_g[order].astype(np.int32)
         self.free = free[order].astype(np.uint8)
+        self.poly = poly[order]
         hs = height[order]
         H = int(hs.max()) + 1
         self.num_levels = H
@@ -299,6 +305,54 @@ class Forest(ForestLayout):
     def set_flags(self, flags):
         a = np.asarray(flags, dtype=np.uint32)
         _lib.check(self._lib.laml_forest_set_flags(self.handle, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+
+    # ---------------- ultrametric constraint ----------------
+    def set_ultrametric(self, model):
+        """Enable the ultrametric M-step with a laml_amd.ultra.HeightModel
+        (None disables it)."""
+        if model is None:
+            _lib.check(self._lib.laml_forest_set_ultrametric(self.handle, None))
+            self._ultra = None
+            return
+        P = self.num_problems
+        H = len(model.cls_level_off) - 1
+        plo = np.zeros(H * P + 1, dtype=np.int32)
+        cp = model.cls_prob
+        lvl = np.zeros(model.num_classes, dtype=np.int64)
+        for h in range(H):
+            lvl[model.cls_level_off[h]:model.cls_level_off[h + 1]] = h
+        key = lvl * P + cp
+        plo[:] = np.searchsorted(key, np.arange(H * P + 1)).astype(np.int32)
+        arrs = dict(plo=plo, up_cls=model.up_cls.astype(np.int32), dn_cls=model.dn_cls.astype(np.int32),
+                    up_off=model.up_off.astype(np.float64), dn_off=model.dn_off.astype(np.float64),
+                    cls_parent=model.cls_parent.astype(np.int32), cls_pedge=model.cls_pedge.astype(np.int32),
+                    inc_off=model.inc_off.astype(np.int32), inc_branch=model.inc_branch.astype(np.int32),
+                    inc_sign=model.inc_sign.astype(np.int8), has_free=model.prob_has_free.astype(np.uint8))
+        u = _lib.UltraDesc()
+        u.num_classes = model.num_classes
+        u.num_levels = H
+        u.level_problem_off = arrs["plo"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        u.up_cls = arrs["up_cls"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        u.dn_cls = arrs["dn_cls"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        u.up_off = arrs["up_off"].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        u.dn_off = arrs["dn_off"].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        u.cls_parent = arrs["cls_parent"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        u.cls_pedge = arrs["cls_pedge"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        u.inc_off = arrs["inc_off"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        u.inc_branch = arrs["inc_branch"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        u.inc_sign = arrs["inc_sign"].ctypes.data_as(ctypes.POINTER(ctypes.c_int8))
+        u.problem_has_free = arrs["has_free"].ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        _lib.check(self._lib.laml_forest_set_ultrametric(self.handle, ctypes.byref(u)))
+        self._ultra = (model, arrs)
+
+    def set_heights(self, t):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        _lib.check(self._lib.laml_forest_set_heights(self.handle, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def ultra_status(self):
+        out = np.zeros(self.num_problems, dtype=np.int32)
+        _lib.check(self._lib.laml_forest_ultra_status(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return out
 
     def labels(self):
         """az-partition labels: (leaf rows, internal rows) int16 flat arrays in

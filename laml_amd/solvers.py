@@ -23,6 +23,7 @@ from ._lib import LAML_OPT_NU, LAML_OPT_PHI
 from .forest import DMAX, DMIN, Forest, ProblemSpec, SiteTable, tree_arrays_from_tree
 from .lca import find_LCAs
 from .tree import read_tree_newick
+from .ultra import HeightModel, UltraInfeasible, ultrametric_constr as _ultrametric_constr
 
 # laml_libs/core_constants.py:3-10
 MIN_LLH = -float("inf")
@@ -128,6 +129,33 @@ class EM_solver:
                              for v in t.traverse_postorder()))
         return tuple(key)
 
+    def ultrametric_constr(self, local_brlen_opt=True):
+        """EM_solver.ultrametric_constr (EM_solver.py:53-89): (M, b)."""
+        return _ultrametric_constr(self.trees, local_brlen_opt=local_brlen_opt)
+
+    def _enable_ultra(self, F, d):
+        """Height-class model of the current tree/fixed set on the device, with
+        strictly feasible start heights from the branch lengths d (level
+        order).  Returns False when the constraints are infeasible."""
+        key = self._forest_key
+        try:
+            if getattr(self, "_ultra_key", None) != key:
+                self._ultra_model = HeightModel(F, d)
+                self._ultra_key = key
+                F.set_ultrametric(self._ultra_model)
+            t0 = self._ultra_model.heights_from_lengths(d, self.dmin, self.dmax)
+            if not self._ultra_model.check_strict(t0, F.free, self.dmin, self.dmax):
+                return False
+        except UltraInfeasible:
+            return False
+        F.set_heights(t0)
+        return True
+
+    def _disable_ultra(self, F):
+        if getattr(self, "_ultra_key", None) is not None:
+            F.set_ultrametric(None)
+            self._ultra_key = None
+
     def _problem_spec(self, flags):
         sites = self._site_table()
         rows = {}
@@ -144,7 +172,10 @@ class EM_solver:
         def free_of(v):
             return not v.polytomy_mark and not getattr(v, 'mark_fixed', False)
 
-        arrays = [tree_arrays_from_tree(t, codes_of, free_of) for t in self.trees]
+        def poly_of(v):
+            return bool(v.polytomy_mark)
+
+        arrays = [tree_arrays_from_tree(t, codes_of, free_of, poly_of) for t in self.trees]
         nobs = 0
         for t in self.trees:
             for v in t.traverse_leaves():
@@ -162,6 +193,7 @@ class EM_solver:
             self._forest = Forest([spec], device=dev, dmin=self.dmin, dmax=self.dmax)
             self._forest_key = key
             self._forest_flags = flags
+            self._ultra_key = None
         elif self._forest_flags != flags:
             self._forest.set_flags([flags])
             self._forest_flags = flags
@@ -250,12 +282,17 @@ class EM_solver:
 
     def Mstep(self, optimize_phi=True, optimize_nu=True, verbose=1, eps_nu=1e-5, eps_s=1e-6,
               ultra_constr_cache=None, local_brlen_opt=True):
-        """EM_solver.Mstep (EM_solver.py:307-459) on the device, after Estep."""
-        if ultra_constr_cache is not None:
-            raise NotImplementedError("ultrametric-constrained M-step is not available on the device path")
+        """EM_solver.Mstep (EM_solver.py:307-459) on the device, after Estep.
+        A non-None ultra_constr_cache selects the ultrametric-constrained
+        branch problem (the height-class model replaces the (M, b) matrix)."""
         flags = (LAML_OPT_PHI if optimize_phi else 0) | (LAML_OPT_NU if optimize_nu else 0)
         F = self._forest_for(flags)
         d, nu, phi = self._device_params(F)
+        if ultra_constr_cache is not None:
+            if not self._enable_ultra(F, d.cpu().numpy()):
+                return False, "d_infeasible"
+        else:
+            self._disable_ultra(F)
         F.mstep(d, nu, phi)
         self._store_params(F, d, nu, phi)
         return True, "optimal"
@@ -263,11 +300,16 @@ class EM_solver:
     def EM_optimization(self, verbose=1, optimize_phi=True, optimize_nu=True, ultra_constr=False, maxIter=1000):
         """EM_solver.EM_optimization (EM_solver.py:461-512), whole loop on
         the device.  Returns (nllh, em_iter, status)."""
-        if ultra_constr:
-            raise NotImplementedError("ultrametric-constrained EM is not available on the device path")
         flags = (LAML_OPT_PHI if optimize_phi else 0) | (LAML_OPT_NU if optimize_nu else 0)
         F = self._forest_for(flags)
         d, nu, phi = self._device_params(F)
+        if ultra_constr:
+            if not self._enable_ultra(F, d.cpu().numpy()):
+                if verbose >= 0:
+                    print("Warning: EM failed to optimize parameters in one Mstep due to infeasible constraints")
+                return float(F.nll(d, nu, phi).item()), 1, "d_infeasible"
+        else:
+            self._disable_ultra(F)
         if verbose >= 0:
             pre = F.nll(d, nu, phi).item()
             print("Initial phi: " + str(self.params.phi) + ". Initial nu: " + str(self.params.nu)

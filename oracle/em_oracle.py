@@ -28,8 +28,10 @@ closed third-party solver.  The oracle replaces it with exact solutions of the
 same concave problems: per-branch bisection on the stationarity condition
 (the problem is separable without the ultrametric constraint) and bisection
 for the 1-D nu problem.  Those solutions are pinned by the KKT conditions in
-tests, not by the reference.  The ultrametric-constrained M-step is not
-restated here (parity unpinned for it in this module).
+tests, not by the reference.  With the ultrametric constraint (the reference's
+equality system, EM_solver.py:53-89, restated in ``ultrametric_constr``) the
+branch problem is solved by scipy's SLSQP in branch-length space; its accuracy
+(~1e-8) is what the device solver is compared against.
 
 Tree handling restates what the reference takes from treeswift (>=1.1.39, not
 installed here): Newick reading, post/pre-order traversal with children in
@@ -550,8 +552,43 @@ class OracleEM:
                     nodes.append(v)
         return rows, nodes
 
-    def mstep(self, optimize_phi=True, optimize_nu=True, eps_nu=1e-5, local_brlen_opt=True):
-        """EM_solver.Mstep (EM_solver.py:307-459) without the ultrametric constraint."""
+    def ultrametric_constr(self, local_brlen_opt=True):
+        """EM_solver.ultrametric_constr (EM_solver.py:53-89): (M, b) over the
+        optimised branches in post-order."""
+        N = self.num_edges - self.num_polytomy_mark
+        if local_brlen_opt:
+            for r in self.trees:
+                N -= sum(1 for v in postorder(r) if v.a["fixed"])
+        constrs = {}
+        idx = 0
+        for r in self.trees:
+            for v in postorder(r):
+                if v.is_leaf():
+                    v.a["cvec"] = [0.0] * N
+                    v.a["const"] = v.edge_length if v.a["fixed"] else 0
+                else:
+                    c1, c2 = v.children
+                    m = tuple(x - y for x, y in zip(c1.a["cvec"], c2.a["cvec"]))
+                    mc = tuple(-x for x in m)
+                    c = c2.a["const"] - c1.a["const"]
+                    if sum(x != 0 for x in m) > 0 and not (m in constrs or mc in constrs):
+                        constrs[m] = c
+                    v.a["cvec"] = c1.a["cvec"]
+                    v.a["const"] = c1.a["const"] + v.edge_length if v.a["fixed"] else c1.a["const"]
+                if not v.a["poly"] and not (v.a["fixed"] and local_brlen_opt):
+                    v.a["cvec"][idx] = 1
+                    idx += 1
+        for r in self.trees[1:]:
+            r0 = self.trees[0]
+            m = tuple(x - y for x, y in zip(r0.a["cvec"], r.a["cvec"]))
+            constrs[m] = r.a["const"] - r0.a["const"]
+        return [list(m) for m in constrs], [constrs[m] for m in constrs]
+
+    def mstep(self, optimize_phi=True, optimize_nu=True, eps_nu=1e-5, local_brlen_opt=True, ultra=None):
+        """EM_solver.Mstep (EM_solver.py:307-459).  ``ultra`` = (M, b) adds the
+        ultrametric equality system; that branch problem is solved with scipy's
+        SLSQP in branch-length space (an algorithm independent of the device
+        solver)."""
         if optimize_phi:
             R, Rt = self.phi_terms()
             phi_star = Rt / (R + Rt)
@@ -563,7 +600,11 @@ class OracleEM:
         nu = self.nu
         use_nu = sum(s[2] for s in rows) > 0 and nu > eps_nu
         use_nu4 = sum(s[4] for s in rows) > 0 and nu > eps_nu
-        d_star = [solve_branch(s, nu, use_nu, use_nu4, self.dmin, self.dmax) for s in rows]
+        if ultra is None:
+            d_star = [solve_branch(s, nu, use_nu, use_nu4, self.dmin, self.dmax) for s in rows]
+        else:
+            d_star = solve_branches_constrained(rows, nu, use_nu, use_nu4, self.dmin, self.dmax, ultra[0], ultra[1],
+                                                [v.edge_length for v in nodes])
         if optimize_nu:
             nu_star = solve_nu(rows, d_star)
         else:
@@ -574,9 +615,11 @@ class OracleEM:
             v.edge_length = d
         return True, "optimal"
 
-    def em_optimization(self, optimize_phi=True, optimize_nu=True, maxIter=1000, fixed_iters=None):
+    def em_optimization(self, optimize_phi=True, optimize_nu=True, maxIter=1000, fixed_iters=None,
+                        ultra_constr=False):
         """EM_solver.EM_optimization (EM_solver.py:461-512).  ``fixed_iters``
         runs exactly that many iterations with no convergence test."""
+        ultra = self.ultrametric_constr() if ultra_constr else None
         pre = self.lineage_llh()
         em_iter = 1
         status = ""
@@ -584,7 +627,7 @@ class OracleEM:
         self.llh_trace = [pre]
         while em_iter <= (maxIter if fixed_iters is None else fixed_iters):
             self.estep()
-            ok, status = self.mstep(optimize_phi=optimize_phi, optimize_nu=optimize_nu)
+            ok, status = self.mstep(optimize_phi=optimize_phi, optimize_nu=optimize_nu, ultra=ultra)
             curr = self.lineage_llh()
             self.llh_trace.append(curr)
             if fixed_iters is None and abs((curr - pre) / pre) < CONV_EPS:
@@ -658,6 +701,38 @@ def solve_branch(s, nu, use_nu, use_nu4, dmin, dmax):
             lo = mid
         else:
             hi = mid
+
+
+def solve_branches_constrained(rows, nu, use_nu, use_nu4, dmin, dmax, M, b, d0):
+    """argmax of the summed branch objectives subject to M d = b and the box,
+    with scipy SLSQP (analytic gradient), from a feasible-ish start."""
+    import numpy as np
+    from scipy.optimize import minimize
+
+    S = np.asarray(rows, dtype=float)
+    A = (nu + 1) * S[:, 0] + nu * S[:, 1] + nu * S[:, 3]
+    S1 = S[:, 1]
+    B = (S[:, 2] if use_nu else 0.0) + (S[:, 4] if use_nu4 else 0.0)
+
+    def f(d):
+        val = -A @ d + S1 @ np.log(-np.expm1(-d))
+        if nu > 0:
+            val = val + B @ np.log(-np.expm1(-nu * d)) if np.any(B) else val
+        return -val
+
+    def g(d):
+        gr = -A + S1 / np.expm1(d)
+        if nu > 0 and np.any(B):
+            gr = gr + B * nu / np.expm1(nu * d)
+        return -gr
+
+    Mm = np.asarray(M, dtype=float).reshape(-1, len(rows))
+    bb = np.asarray(b, dtype=float)
+    cons = [{"type": "eq", "fun": lambda d: Mm @ d - bb, "jac": lambda d: Mm}] if len(bb) else []
+    x0 = np.clip(np.asarray(d0, dtype=float), dmin * 1.01, dmax * 0.99)
+    res = minimize(f, x0, jac=g, method="SLSQP", bounds=[(dmin, dmax)] * len(rows), constraints=cons,
+                   options={"ftol": 1e-16, "maxiter": 5000})
+    return list(res.x)
 
 
 def nu_grad(rows, d, nu):

@@ -236,3 +236,71 @@ def test_deep_tree_em_from_uniform_start():
     print("deep em", (-trace).cpu().numpy(), o.llh_trace, flush=True)
     np.testing.assert_allclose((-trace).cpu().numpy(), o.llh_trace, rtol=NLL_RTOL)
     assert abs(nu_t.item() - o.nu) < 1e-6 and abs(phi_t.item() - o.phi) < 1e-6
+
+
+def _ultra_pair(T, msa, Q, nu, phi, fixed_labels=()):
+    o, s = _pair(T, msa, Q, nu, phi)
+    o.az_partition()
+    return o, s
+
+
+@pytest.mark.parametrize("name", ["test4", "syn250x30"])
+def test_ultrametric_mstep_matches_slsqp(name):
+    """One constrained M-step: device barrier-Newton vs scipy SLSQP on the
+    restated reference constraint matrix; the device result satisfies the
+    equality system to rounding."""
+    T, msa, Q, nu, phi = DATASETS[name]()
+    o, s = _pair(T, msa, Q, nu, phi)
+    o.az_partition()
+    o.estep()
+    M, b = o.ultrametric_constr()
+    o.mstep(ultra=(M, b))
+    s.Estep()
+    ok, status = s.Mstep(ultra_constr_cache=s.ultrametric_constr())
+    assert ok and status == "optimal"
+    assert s._forest.ultra_status()[0] == 0
+    dg = np.array([v.edge_length for v in _post_nodes(s)])
+    do = np.array([v.edge_length for v in postorder(o.trees[0])])
+    Ms, bs = s.ultrametric_constr()
+    print("ultra", name, np.abs(dg - do).max(), np.abs(np.array(Ms) @ dg - np.array(bs)).max(), flush=True)
+    assert np.abs(np.array(Ms) @ dg - np.array(bs)).max() < 1e-10
+    assert (dg >= 0.005 - 1e-12).all() and (dg <= 10 + 1e-12).all()
+    np.testing.assert_allclose(dg, do, atol=1e-6)
+    assert abs(s.params.nu - o.nu) < 1e-6
+
+
+def test_ultrametric_em_matches_oracle():
+    T, msa, Q, nu, phi = DATASETS["test4"]()
+    o, s = _pair(T, msa, Q, 0.1, 0.05)
+    iters = 5
+    o.az_partition()
+    onll, _, _ = o.em_optimization(fixed_iters=iters, ultra_constr=True)
+    F = s._forest_for()
+    d, nu_t, phi_t = s._device_params(F)
+    assert s._enable_ultra(F, d.cpu().numpy())
+    trace = torch.zeros(iters + 1, dtype=torch.float64, device=F.device)
+    nll, it = F.em(d, nu_t, phi_t, iters, 0.0, trace=trace)
+    print("ultra em", (-trace).cpu().numpy(), o.llh_trace, flush=True)
+    np.testing.assert_allclose((-trace).cpu().numpy(), o.llh_trace, rtol=1e-8)
+    s._store_params(F, d, nu_t, phi_t)
+    dg = np.array([v.edge_length for v in _post_nodes(s)])
+    do = np.array([v.edge_length for v in postorder(o.trees[0])])
+    np.testing.assert_allclose(dg, do, atol=1e-6)
+
+
+def test_ultrametric_optimize_drop_in():
+    """optimize(ultra_constr=True) through the drop-in surface: the fitted tree
+    is ultrametric (all root-to-leaf paths equal) and its NLL is no better than
+    the unconstrained optimum."""
+    T, msa, Q = load_test4()
+    s = EM_solver([T], {"charMtrx": msa}, {"Q": Q}, {"phi": 0, "nu": 0})
+    nllh_u, status = s.optimize(initials=1, random_seeds=1221, verbose=-1, ultra_constr=True)
+    assert status == "optimal"
+    tr = s.trees[0]
+    dist = {}
+    for v in tr.traverse_preorder():
+        dist[v] = (v.edge_length or 0.0) + (dist[v.parent] if v.parent is not None else 0.0)
+    depths = [dist[v] for v in tr.traverse_leaves()]
+    print("ultra optimize", nllh_u, min(depths), max(depths), flush=True)
+    assert max(depths) - min(depths) < 1e-8
+    assert nllh_u >= 1007.009158767873 * (1 - 1e-6)
