@@ -96,6 +96,19 @@ void laml_forest_destroy(laml_forest* f);
  * (EM_solver.py:514-523). */
 int laml_forest_set_flags(laml_forest* f, const uint32_t* flags_host);
 
+/* Level checkpointing of the inside vectors: with levels = L > 0 only the
+ * inside vectors of nodes whose parent lies in a higher block of L levels
+ * (and the roots) are kept; the outside pass walks the blocks top-down and
+ * recomputes each block's inside vectors in one reusable working area before
+ * using them.  levels = 0 restores full storage.  Inside state must be
+ * recomputed (laml_nll) after a change.
+ * No counterpart in the reference (its inside values live in Python
+ * objects); it bounds the memory of the adjoint for very large trees. */
+int laml_forest_set_checkpoint(laml_forest* f, int32_t levels);
+
+/* Rows (node x site) of inside-vector storage currently allocated. */
+int64_t laml_forest_inside_rows(laml_forest* f);
+
 /* Per-node az-partition labels, int16 [num_nodes * K] row-major (0 = 'z',
  * -1 = '?', j>0 = local edit j), copied to host.  Replaces reading node.alpha
  * after ML_solver.az_partition (ML_solver.py:125-154). */
@@ -126,6 +139,18 @@ int laml_estep(laml_forest* f, const double* d, const double* nu, const double* 
  * Replaces: EM_solver.Estep_out_llh + __out_alpha_up__ + Estep_posterior
  * (EM_solver.py:143-300). */
 int laml_outside(laml_forest* f, const double* phi, double* counts, double* rtilde, void* stream);
+
+/* Gradient of the negative log-likelihood with respect to every branch length
+ * (device [num_nodes]), nu and phi (device [num_problems] each): the outside
+ * pass run as the adjoint of the inside pass left resident by the preceding
+ * laml_nll with the same parameters, then the Fisher identity on the raw
+ * expected transition counts.  Defined for nu > 0 and 0 < phi < 1.
+ * No counterpart in the reference, which has no gradient; its expected
+ * counts are those of EM_solver.Estep_posterior (EM_solver.py:247-300) and the
+ * score terms those of the M-step objective (EM_solver.py:349-357) and of
+ * fast_em/laml.py:62-78. */
+int laml_grad(laml_forest* f, const double* d, const double* nu, const double* phi, double* grad_d,
+              double* grad_nu, double* grad_phi, void* stream);
 
 /* M-step, in place on d, nu, phi (device): phi closed form, per-branch exact
  * maximisation of the branch-length objective over [dmin, dmax] for free

@@ -78,11 +78,14 @@ SIGNATURES = {
                                           ctypes.POINTER(ctypes.c_void_p)]),
     "laml_forest_validate": (ctypes.c_int, [ctypes.POINTER(ForestDesc)]),
     "laml_forest_destroy": (None, [ctypes.c_void_p]),
+    "laml_forest_set_checkpoint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "laml_forest_inside_rows": (ctypes.c_int64, [ctypes.c_void_p]),
     "laml_forest_set_flags": (ctypes.c_int, [ctypes.c_void_p, _u32p]),
     "laml_forest_labels": (ctypes.c_int, [ctypes.c_void_p, _i16p, ctypes.c_int64]),
     "laml_nll": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 + [ctypes.c_void_p]),
     "laml_estep": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 6 + [ctypes.c_void_p]),
     "laml_outside": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 3 + [ctypes.c_void_p]),
+    "laml_grad": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 6 + [ctypes.c_void_p]),
     "laml_mstep": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 + [ctypes.c_void_p]),
     "laml_em": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                ctypes.c_int32, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,

@@ -35,6 +35,14 @@ struct laml_forest {
   int device = 0;
   int W = 64;
   int h_top = 1 << 30;  // first level handled by the fused top-level kernels
+  int h_top_default = 1 << 30;
+  int ckpt_L = 0;       // level-checkpoint segment length (0: all inside vectors stored)
+  // host copies for re-layout of the inside vectors
+  std::vector<int32_t> h_parent, h_K;
+  std::vector<int64_t> h_row_off;
+  void* beta_alloc = nullptr;
+  void* bE_alloc = nullptr;
+  int64_t beta_rows = 0;
   DevForest F{};
   std::vector<int32_t> level_off;  // host copy
   std::vector<int32_t> tree_nsites;
@@ -100,6 +108,8 @@ struct laml_forest {
   ~laml_forest() {
     for (void* p : allocs) (void)hipFree(p);
     for (void* p : ultra_allocs) (void)hipFree(p);
+    if (beta_alloc) (void)hipFree(beta_alloc);
+    if (bE_alloc) (void)hipFree(bE_alloc);
   }
 };
 
@@ -121,6 +131,7 @@ struct Derived {
   int64_t leaf_rows = 0, int_rows = 0;
   std::vector<int32_t> prob_tree_off, prob_node_off, prob_nodes, prob_int_off, prob_nfree, node_pos, tree_site_base;
   std::vector<int32_t> prob_blk_off;
+  std::vector<double> prob_nobs_phi;
   int max_k = 0;
 };
 
@@ -219,6 +230,15 @@ int validate(const laml_forest_desc* d, Derived& D) {
       if (c < -2 || c > m) return fail(LAML_EINVAL, "leaf code out of range for its site");
     }
   }
+  // entries observed as 0 or an edit, per problem (phi gradient)
+  D.prob_nobs_phi.assign(P, 0.0);
+  for (int v = 0; v < n; ++v) {
+    const int t = d->tree_of[v];
+    const int K = d->tree_nsites[t];
+    double c = 0.0;
+    for (int k = 0; k < K; ++k) c += d->leaf_codes[row_off[v] + k] >= 0 ? 1.0 : 0.0;
+    D.prob_nobs_phi[d->tree_problem[t]] += c;
+  }
   // nodes grouped by problem
   prob_node_off.assign(P + 1, 0);
   for (int v = 0; v < N; ++v) prob_node_off[d->tree_problem[d->tree_of[v]] + 1]++;
@@ -289,6 +309,7 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
     int ht = d->num_levels;
     while (ht - 1 >= 1 && d->level_off[ht] - d->level_off[ht - 1] <= top_max) --ht;
     f->h_top = ht;
+    f->h_top_default = ht;
   }
   f->leaf_rows = leaf_rows;
   f->int_rows = int_rows;
@@ -328,6 +349,7 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   UP(prob_flags, d->problem_flags, P);
   UP(prob_nobs, d->problem_nobs, P);
   UP(prob_phimin, d->problem_phimin, P);
+  UP(prob_nobs_phi, D.prob_nobs_phi.data(), P);
   UP(site_qoff, d->site_qoff, S + 1);
   UP(prior, d->prior, d->site_qoff[S]);
   UP(codes, d->leaf_codes, leaf_rows);
@@ -344,8 +366,22 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   laml::NuState* nu_state;
   AL(labels, int16_t, int_rows);
   AL(coef, laml::Coef, N);
-  AL(beta, double2, int_rows);
-  AL(bE, int32_t, int_rows);
+  {
+    void* q = nullptr;
+    if (hipMalloc(&q, sizeof(double2) * std::max<int64_t>(int_rows, 1)) != hipSuccess) {
+      delete f;
+      return fail(LAML_ENOMEM, "hipMalloc beta");
+    }
+    f->beta_alloc = q;
+    beta = static_cast<double2*>(q);
+    if (hipMalloc(&q, sizeof(int32_t) * std::max<int64_t>(int_rows, 1)) != hipSuccess) {
+      delete f;
+      return fail(LAML_ENOMEM, "hipMalloc bE");
+    }
+    f->bE_alloc = q;
+    bE = static_cast<int32_t*>(q);
+    f->beta_rows = int_rows;
+  }
   AL(alpha, double4, int_rows);
   AL(aE, int32_t, int_rows);
   AL(aG, int16_t, int_rows);
@@ -367,6 +403,11 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   AL(f->em.pre, double, P);
 #undef AL
   F.labels = labels;
+  F.brow = F.row_off;
+  f->h_parent.assign(d->parent, d->parent + N);
+  f->h_row_off = row_off;
+  f->h_K.resize(N);
+  for (int v = 0; v < N; ++v) f->h_K[v] = d->tree_nsites[d->tree_of[v]];
   F.coef = coef;
   F.beta = beta;
   F.bE = bE;
@@ -435,6 +476,79 @@ int laml_forest_labels(laml_forest* f, int16_t* labels_host, int64_t capacity) {
   std::memcpy(labels_host + f->leaf_rows, lab.data(), sizeof(int16_t) * f->int_rows);
   return LAML_OK;
 }
+
+int laml_forest_set_checkpoint(laml_forest* f, int32_t levels) {
+  if (!f) return fail(LAML_EINVAL, "null forest");
+  if (levels < 0) return fail(LAML_EINVAL, "negative checkpoint length");
+  DeviceGuard guard(f->device);
+  LAML_HIP(hipDeviceSynchronize());
+  const int N = f->F.num_nodes, n = f->F.num_leaves;
+  const int H = (int)f->level_off.size() - 1;
+  std::vector<int64_t> brow(N, 0);
+  int64_t rows = 0;
+  if (levels == 0) {
+    brow = f->h_row_off;
+    for (int v = n; v < N; ++v) rows += f->h_K[v];
+    f->h_top = f->h_top_default;
+  } else {
+    std::vector<int32_t> lev(N, 0);
+    for (int h = 0; h < H; ++h)
+      for (int v = f->level_off[h]; v < f->level_off[h + 1]; ++v) lev[v] = h;
+    auto seg = [&](int v) { return (lev[v] - 1) / levels; };
+    const int nseg = (H - 1 + levels - 1) / levels;
+    std::vector<char> exported(N, 0);
+    for (int v = n; v < N; ++v) {
+      const int pa = f->h_parent[v];
+      exported[v] = (pa < 0 || seg(pa) != seg(v)) ? 1 : 0;
+    }
+    for (int v = n; v < N; ++v)
+      if (exported[v]) {
+        brow[v] = rows;
+        rows += f->h_K[v];
+      }
+    const int64_t base = rows;
+    std::vector<int64_t> seg_rows(std::max(nseg, 1), 0);
+    int64_t work = 0;
+    for (int v = n; v < N; ++v)
+      if (!exported[v]) {
+        const int sg = seg(v);
+        brow[v] = base + seg_rows[sg];
+        seg_rows[sg] += f->h_K[v];
+        work = std::max(work, seg_rows[sg]);
+      }
+    rows = base + work;
+    f->h_top = H;  // the fused top-level kernels do not cross segments
+  }
+  // new inside-vector storage
+  void* nb = nullptr;
+  void* ne = nullptr;
+  if (hipMalloc(&nb, sizeof(double2) * std::max<int64_t>(rows, 1)) != hipSuccess)
+    return fail(LAML_ENOMEM, "hipMalloc beta");
+  if (hipMalloc(&ne, sizeof(int32_t) * std::max<int64_t>(rows, 1)) != hipSuccess) {
+    (void)hipFree(nb);
+    return fail(LAML_ENOMEM, "hipMalloc bE");
+  }
+  int64_t* db = nullptr;
+  int rc = f->alloc(&db, N);
+  if (rc) {
+    (void)hipFree(nb);
+    (void)hipFree(ne);
+    return rc;
+  }
+  LAML_HIP(hipMemcpy(db, brow.data(), sizeof(int64_t) * N, hipMemcpyHostToDevice));
+  if (f->beta_alloc) (void)hipFree(f->beta_alloc);
+  if (f->bE_alloc) (void)hipFree(f->bE_alloc);
+  f->beta_alloc = nb;
+  f->bE_alloc = ne;
+  f->beta_rows = rows;
+  f->F.beta = static_cast<double2*>(nb);
+  f->F.bE = static_cast<int32_t*>(ne);
+  f->F.brow = db;
+  f->ckpt_L = levels;
+  return LAML_OK;
+}
+
+int64_t laml_forest_inside_rows(laml_forest* f) { return f ? f->beta_rows : -1; }
 
 int laml_forest_set_ultrametric(laml_forest* f, const laml_ultra_desc* u) {
   if (!f) return fail(LAML_EINVAL, "null forest");
@@ -554,7 +668,7 @@ int laml_estep(laml_forest* f, const double* d, const double* nu, const double* 
   laml::run_root(F, f->W, nullptr, nullptr, s);
   EmState none{};
   laml::run_llh(F, nll, none, 0, 0, 0.0, s);
-  laml::run_outside(F, f->level_off, f->W, f->h_top, nullptr, s);
+  (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, nullptr, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, nullptr, s));
   laml::run_counts(F, counts, s);
   laml::run_rtilde(F, rtilde, nullptr, s);
   return check_launch();
@@ -567,9 +681,22 @@ int laml_outside(laml_forest* f, const double* phi, double* counts, double* rtil
   DevForest F = f->F;
   F.phi = phi;
   laml::run_root(F, f->W, nullptr, nullptr, s);
-  laml::run_outside(F, f->level_off, f->W, f->h_top, nullptr, s);
+  (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, nullptr, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, nullptr, s));
   laml::run_counts(F, counts, s);
   laml::run_rtilde(F, rtilde, nullptr, s);
+  return check_launch();
+}
+
+int laml_grad(laml_forest* f, const double* d, const double* nu, const double* phi, double* grad_d,
+              double* grad_nu, double* grad_phi, void* stream) {
+  if (!f || !d || !nu || !phi || !grad_d || !grad_nu || !grad_phi) return fail(LAML_EINVAL, "null argument");
+  DeviceGuard guard(f->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevForest F = f->F;
+  F.phi = phi;
+  laml::run_root(F, f->W, nullptr, nullptr, s);
+  (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, nullptr, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, nullptr, s));
+  laml::run_grad(F, d, nu, phi, grad_d, grad_nu, grad_phi, s);
   return check_launch();
 }
 
@@ -603,7 +730,7 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
     laml::run_root(F, f->W, nullptr, em.done, s);
     laml::run_llh(F, nll, em, step, max_iter, tol, s);
     if (step < max_iter) {
-      laml::run_outside(F, f->level_off, f->W, f->h_top, em.done, s);
+      (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, em.done, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, em.done, s));
       laml::run_mstep(F, nullptr, nullptr, d, nu, phi, em.done, s, f->ultra_on ? &f->U : nullptr);
     }
     int rc = check_launch();

@@ -65,9 +65,11 @@ __global__ void __launch_bounds__(256) k_labels(DevForest F, int beg, int cnt) {
   F.labels[F.row_off[v] + k] = (int16_t)combine_label(la, lb);
 }
 
-// load the inside state of child c at site k
-__device__ __forceinline__ void load_child(const DevForest& F, bool leaf, int64_t r, int k, double phi,
-                                           int& lab, double& b0, double& bA, int& E, int& code) {
+// load the inside state of node c at site k (labels/codes by row_off, the
+// inside vector by brow, which differs from row_off under level checkpointing)
+__device__ __forceinline__ void load_child(const DevForest& F, int c, bool leaf, int k, double phi, int& lab,
+                                           double& b0, double& bA, int& E, int& code) {
+  const int64_t r = F.row_off[c];
   if (leaf) {
     code = F.codes[r + k];
     leaf_beta(code, phi, lab, b0, bA);
@@ -75,10 +77,11 @@ __device__ __forceinline__ void load_child(const DevForest& F, bool leaf, int64_
   } else {
     code = 0;
     lab = F.labels[r + k];
-    const double2 bb = F.beta[r + k];
+    const int64_t br = F.brow[c];
+    const double2 bb = F.beta[br + k];
     b0 = bb.x;
     bA = bb.y;
-    E = F.bE[r + k];
+    E = F.bE[br + k];
   }
 }
 
@@ -96,8 +99,8 @@ __device__ __forceinline__ void inside_node(const DevForest& F, int v, int k, co
   const int lv = F.labels[rv + k];
   int la, lb, Ea, Eb, ca, cb;
   double a0, aA, b0, bA;
-  load_child(F, c1 < F.num_leaves, F.row_off[c1], k, phi, la, a0, aA, Ea, ca);
-  load_child(F, c2 < F.num_leaves, F.row_off[c2], k, phi, lb, b0, bA, Eb, cb);
+  load_child(F, c1, c1 < F.num_leaves, k, phi, la, a0, aA, Ea, ca);
+  load_child(F, c2, c2 < F.num_leaves, k, phi, lb, b0, bA, Eb, cb);
   const int qo = F.site_qoff[sb + k] - 1;
   const double qa = la > 0 ? F.prior[qo + la] : 1.0;
   const double qb = lb > 0 ? F.prior[qo + lb] : 1.0;
@@ -109,8 +112,9 @@ __device__ __forceinline__ void inside_node(const DevForest& F, int v, int k, co
   // cannot produce an infinite scale)
   const int sh = min(-exp_of(fmax(B0, BA)), 1000);
   const double sc = ldexp(1.0, sh);
-  F.beta[rv + k] = make_double2(B0 * sc, BA * sc);
-  F.bE[rv + k] = ea.e + eb.e - sh;
+  const int64_t bv = F.brow[v];
+  F.beta[bv + k] = make_double2(B0 * sc, BA * sc);
+  F.bE[bv + k] = ea.e + eb.e - sh;
 }
 
 template <int W>
@@ -159,7 +163,7 @@ __global__ void __launch_bounds__(256) k_root(DevForest F, double* __restrict__ 
     const int64_t rr = F.row_off[r];
     int lab, E, code;
     double b0, bA;
-    load_child(F, leaf, rr, k, phi, lab, b0, bA, E, code);
+    load_child(F, r, leaf, k, phi, lab, b0, bA, E, code);
     const int qo = F.site_qoff[sb + k] - 1;
     const double q = lab > 0 ? F.prior[qo + lab] : 1.0;
     const Eps er = edge(lab, b0, bA, E, q, cf);
@@ -240,7 +244,7 @@ __device__ __forceinline__ void outside_node(const DevForest& F, int u, int chun
     Eps ep[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      load_child(F, lf[i], rc[i], k, phi, lab[i], b0[i], bA[i], E[i], code[i]);
+      load_child(F, ch[i], lf[i], k, phi, lab[i], b0[i], bA[i], E[i], code[i]);
       q[i] = lab[i] > 0 ? F.prior[qo + lab[i]] : 1.0;
       ep[i] = edge(lab[i], b0[i], bA[i], E[i], q[i], cf[i]);
     }
@@ -655,6 +659,59 @@ __global__ void __launch_bounds__(kNuThreads) k_nu_eval(DevForest F, const doubl
 }
 
 // ---------------------------------------------------------------------------
+// Gradient of the negative log-likelihood (Fisher identity: the gradient of
+// log L is the posterior expectation of the complete-data score).  With raw
+// per-branch expected counts S0..S4 from the outside pass:
+//   dlogL/dd_v  = -(1+nu) S0 + S1 (1/expm1(d) - nu) + (S2+S4) nu/expm1(nu d) - nu S3
+//   dlogL/dnu   = sum_v d_v (-S0 - S1 - S3 + (S2+S4)/expm1(nu d_v))
+//   dlogL/dphi  = -n_obs/(1-phi) + Rt/phi
+// n_obs counts leaf entries observed as 0 or an edit (an explicit -1 entry
+// does not depend on phi); Rt sums P(leaf not silenced | D) over '?' entries.
+// ---------------------------------------------------------------------------
+__global__ void k_grad_branch(DevForest F, const double* __restrict__ d, const double* __restrict__ nu,
+                              double* __restrict__ grad_d) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= F.num_nodes) return;
+  const int t = F.tree_of[v];
+  const int p = F.tree_prob[t];
+  const int C = (F.tree_nsites[t] + F.chunk_w - 1) / F.chunk_w;
+  double s[5] = {0, 0, 0, 0, 0};
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s[j] += F.cpart[((int64_t)v * F.max_chunks + c) * 5 + j];
+  const double dv = d[v], x = nu[p];
+  double g = -(1.0 + x) * s[0] + s[1] * (1.0 / expm1(dv) - x) - x * s[3];
+  if (s[2] + s[4] != 0.0) g += (s[2] + s[4]) * x / expm1(x * dv);
+  grad_d[v] = -g;
+  F.nu_a[F.node_pos[v]] = s[0] + s[1] + s[3];
+  F.nu_b[F.node_pos[v]] = s[2] + s[4];
+  F.nu_d[F.node_pos[v]] = dv;
+}
+
+__global__ void __launch_bounds__(kRedThreads) k_grad_problem(DevForest F, const double* __restrict__ nu,
+                                                              const double* __restrict__ phi,
+                                                              double* __restrict__ grad_nu,
+                                                              double* __restrict__ grad_phi) {
+  __shared__ double sh[kRedThreads / 64];
+  const int p = blockIdx.x;
+  const double x = nu[p];
+  double g = 0.0;
+  for (int i = F.prob_node_off[p] + threadIdx.x; i < F.prob_node_off[p + 1]; i += kRedThreads) {
+    const double dv = F.nu_d[i];
+    const double b = F.nu_b[i];
+    g -= dv * F.nu_a[i];
+    if (b != 0.0) g += b * dv / expm1(x * dv);
+  }
+  g = block_sum<kRedThreads>(g, sh);
+  const double rt = problem_rtilde(F, p, sh);
+  if (threadIdx.x == 0) {
+    grad_nu[p] = -g;
+    const double ph = phi[p];
+    grad_phi[p] = F.prob_nobs_phi[p] / (1.0 - ph) - (rt != 0.0 ? rt / ph : 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Ultrametric branch-length M-step (EM_solver.py:349-366 with the equality
 // system of EM_solver.py:53-89).  Heights of the variable height classes are
 // the unknowns, d_v = t(parent side) - t(child side).  A log-barrier keeps
@@ -906,6 +963,36 @@ void run_root(const DevForest& F, int W, double* site_llh_out, const int32_t* do
   });
 }
 
+static void inside_levels(const DevForest& F, const std::vector<int32_t>& lvl, int W, int a, int b,
+                          const int32_t* done, hipStream_t s) {
+  for (int h = a; h < b; ++h) {
+    const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
+    if (cnt <= 0) continue;
+    LAML_DISPATCH_W(W, {
+      hipLaunchKernelGGL(k_inside<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt, done);
+    });
+  }
+}
+
+void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W, int L, const int32_t* done,
+                      hipStream_t s) {
+  const int H = (int)lvl.size() - 1;
+  const int nseg = (H - 1 + L - 1) / L;  // internal levels 1..H-1
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int a = 1 + sg * L, b = std::min(H, 1 + (sg + 1) * L);
+    // recompute the segment's inside vectors (working area), then its
+    // outside levels from the top of the segment down
+    if (sg != nseg - 1) inside_levels(F, lvl, W, a, b, done, s);
+    for (int h = b - 1; h >= a; --h) {
+      const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
+      if (cnt <= 0) continue;
+      LAML_DISPATCH_W(W, {
+        hipLaunchKernelGGL(k_outside<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt, done);
+      });
+    }
+  }
+}
+
 void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const int32_t* done,
                  hipStream_t s) {
   const int H = (int)lvl.size() - 1;
@@ -930,6 +1017,12 @@ void run_counts(const DevForest& F, double* counts, hipStream_t s) {
 
 void run_llh(const DevForest& F, double* nll, EmState em, int step, int max_iter, double tol, hipStream_t s) {
   hipLaunchKernelGGL(k_llh, dim3(F.num_problems), dim3(256), 0, s, F, nll, em, step, max_iter, tol);
+}
+
+void run_grad(const DevForest& F, const double* d, const double* nu, const double* phi, double* grad_d,
+              double* grad_nu, double* grad_phi, hipStream_t s) {
+  hipLaunchKernelGGL(k_grad_branch, dim3((F.num_nodes + 255) / 256), dim3(256), 0, s, F, d, nu, grad_d);
+  hipLaunchKernelGGL(k_grad_problem, dim3(F.num_problems), dim3(kRedThreads), 0, s, F, nu, phi, grad_nu, grad_phi);
 }
 
 void run_rtilde(const DevForest& F, double* rtilde, const int32_t* done, hipStream_t s) {

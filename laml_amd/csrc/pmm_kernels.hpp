@@ -28,6 +28,7 @@ struct DevForest {
   const int32_t* child1;
   const int32_t* tree_of;
   const int64_t* row_off;
+  const int64_t* brow;  // [N] row of the node's inside vector (beta, bE)
   const uint8_t* free_branch;
   // trees / problems
   const int32_t* tree_root;
@@ -45,6 +46,7 @@ struct DevForest {
   const uint32_t* prob_flags;
   const double* prob_nobs;
   const double* prob_phimin;
+  const double* prob_nobs_phi;  // [P] leaf entries observed as 0 or an edit
   // sites
   const int32_t* site_qoff;  // 1-based local state j -> prior[site_qoff[s] + j - 1]
   const double* prior;
@@ -114,8 +116,12 @@ void run_root(const DevForest& F, int W, double* site_llh_out, const int32_t* do
 void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const int32_t* done,
                  hipStream_t s);
 void run_counts(const DevForest& F, double* counts, hipStream_t s);
+void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W, int L, const int32_t* done,
+                      hipStream_t s);
 void run_llh(const DevForest& F, double* nll, EmState em, int step, int max_iter, double tol, hipStream_t s);
 void run_rtilde(const DevForest& F, double* rtilde, const int32_t* done, hipStream_t s);
+void run_grad(const DevForest& F, const double* d, const double* nu, const double* phi, double* grad_d,
+              double* grad_nu, double* grad_phi, hipStream_t s);
 void run_mstep(const DevForest& F, const double* counts, const double* rtilde, double* d, double* nu, double* phi,
                const int32_t* done, hipStream_t s, const UltraDev* U = nullptr);
 

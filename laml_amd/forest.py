@@ -367,6 +367,7 @@ class Forest(ForestLayout):
 
     # ---------------- calls (device tensors in, device tensors out) -------
     def nll(self, d, nu, phi, site_llh=False):
+        self.state_token = getattr(self, "state_token", 0) + 1
         out = self.nll_buf
         sl = self.site_llh if site_llh else None
         _lib.check(self._lib.laml_nll(self.handle, _lib.ptr(d), _lib.ptr(nu), _lib.ptr(phi), _lib.ptr(out),
@@ -383,6 +384,30 @@ class Forest(ForestLayout):
         _lib.check(self._lib.laml_outside(self.handle, _lib.ptr(phi), _lib.ptr(self.counts), _lib.ptr(self.rtilde),
                                           self.stream()))
         return self.counts, self.rtilde
+
+    def grad(self, d, nu, phi):
+        """Gradient of each problem's NLL w.r.t. d (level order), nu and phi,
+        by the outside pass over the inside state of the preceding nll()."""
+        gd = torch.empty_like(d)
+        gnu = torch.empty_like(nu)
+        gphi = torch.empty_like(phi)
+        _lib.check(self._lib.laml_grad(self.handle, _lib.ptr(d), _lib.ptr(nu), _lib.ptr(phi), _lib.ptr(gd),
+                                       _lib.ptr(gnu), _lib.ptr(gphi), self.stream()))
+        return gd, gnu, gphi
+
+    def set_checkpoint(self, levels):
+        """Keep inside vectors only at level-block boundaries (0: keep all)."""
+        _lib.check(self._lib.laml_forest_set_checkpoint(self.handle, int(levels)))
+        self.state_token = getattr(self, "state_token", 0) + 1
+
+    def inside_rows(self):
+        return int(self._lib.laml_forest_inside_rows(self.handle))
+
+    def node_problem(self):
+        """Problem index of every node (device int64 tensor, cached)."""
+        if getattr(self, "_node_prob_t", None) is None:
+            self._node_prob_t = torch.as_tensor(self.tree_prob[self.tree_of].astype(np.int64), device=self.device)
+        return self._node_prob_t
 
     def mstep(self, d, nu, phi, counts=None, rtilde=None):
         c = self.counts if counts is None else counts
