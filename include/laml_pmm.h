@@ -203,7 +203,8 @@ int laml_forest_set_ultrametric(laml_forest* f, const laml_ultra_desc* u);
 int laml_forest_set_heights(laml_forest* f, const double* heights_host);
 
 /* Per-problem status of the last ultrametric M-step (0 ok, 1 line search
- * stalled), host int32 [num_problems]. */
+ * stalled, 2 singular Newton system; heights are kept at the last accepted
+ * point in both failure cases), host int32 [num_problems]. */
 int laml_forest_ultra_status(laml_forest* f, int32_t* status_host);
 
 /* Text of the last error on this thread ("" if none). */

@@ -603,6 +603,7 @@ int laml_forest_set_ultrametric(laml_forest* f, const laml_ultra_desc* u) {
   UA(t, double, V);
   UA(dt, double, V);
   UA(Dd, double, V);
+  UA(Gg, double, V);
   UA(r, double, V);
   UA(g, double, V);
   UA(gp, double, N);
@@ -614,6 +615,7 @@ int laml_forest_set_ultrametric(laml_forest* f, const laml_ultra_desc* u) {
 #undef UA
   LAML_HIP(hipMemset(U.status, 0, sizeof(int32_t) * P));
   LAML_HIP(hipMemset(U.t, 0, sizeof(double) * (V > 0 ? V : 1)));
+  LAML_HIP(hipMemset(U.dt, 0, sizeof(double) * (V > 0 ? V : 1)));
   f->ultra_on = true;
   return LAML_OK;
 }

@@ -737,8 +737,13 @@ __device__ __forceinline__ double block_min(double x, double* sh) {
   return s;
 }
 
-__device__ __forceinline__ double ult_height(const UltraDev& U, int c, double off, double a) {
-  return c >= 0 ? U.t[c] + a * U.dt[c] + off : off;
+__device__ __forceinline__ double ult_height(const UltraDev& U, int c, double off) {
+  return c >= 0 ? U.t[c] + off : off;
+}
+
+// height at t + a dt (only called with a finite direction)
+__device__ __forceinline__ double ult_height_at(const UltraDev& U, int c, double off, double a) {
+  return c >= 0 ? fma(a, U.dt[c], U.t[c]) + off : off;
 }
 
 // branch objective (barrier excluded): value, first and second derivative
@@ -763,7 +768,8 @@ __device__ double ult_phi(const DevForest& F, const UltraDev& U, int nb, int ne,
   for (int i = nb + threadIdx.x; i < ne; i += kUltraThreads) {
     const int v = F.prob_nodes[i];
     if (!F.free_branch[v]) continue;
-    const double dv = ult_height(U, U.up_cls[v], U.up_off[v], a) - ult_height(U, U.dn_cls[v], U.dn_off[v], a);
+    const double dv =
+        ult_height_at(U, U.up_cls[v], U.up_off[v], a) - ult_height_at(U, U.dn_cls[v], U.dn_off[v], a);
     const double lo = dv - F.dmin, hi = F.dmax - dv;
     if (!(lo > 0.0 && hi > 0.0)) {
       s = -INFINITY;
@@ -797,7 +803,7 @@ __global__ void __launch_bounds__(kUltraThreads) k_ultra(DevForest F, UltraDev U
       for (int i = nb + threadIdx.x; i < ne; i += kUltraThreads) {
         const int v = F.prob_nodes[i];
         if (!F.free_branch[v]) continue;
-        const double dv = ult_height(U, U.up_cls[v], U.up_off[v], 0.0) - ult_height(U, U.dn_cls[v], U.dn_off[v], 0.0);
+        const double dv = ult_height(U, U.up_cls[v], U.up_off[v]) - ult_height(U, U.dn_cls[v], U.dn_off[v]);
         const double lo = dv - dmin, hi = dmax - dv;
         double f, g, h;
         ult_obj(U.bA[v], U.bS1[v], U.bB[v], nu0, dv, f, g, h);
@@ -805,11 +811,16 @@ __global__ void __launch_bounds__(kUltraThreads) k_ultra(DevForest F, UltraDev U
         U.w[v] = -h + mu * (1.0 / (lo * lo) + 1.0 / (hi * hi));
       }
       __syncthreads();
-      // eliminate classes bottom-up
+      // eliminate classes bottom-up.  The eliminated diagonal of class c is
+      // D_c = G_c + w(edge to the parent class), where G_c, the conductance of
+      // c's subtree to the pinned heights, sums positive terms only: pinned
+      // neighbours add w, a child class k adds the series conductance
+      // w G_k / (w + G_k).  (Forming D_c as sum(w) - sum(w^2 / D_k) cancels to
+      // zero when one branch's barrier weight dominates.)
       for (int lv = 0; lv < H; ++lv) {
         const int cb = U.plo[lv * P + p], ce = U.plo[lv * P + p + 1];
         for (int c = cb + threadIdx.x; c < ce; c += kUltraThreads) {
-          double gg = 0.0, L = 0.0, rr = 0.0;
+          double gg = 0.0, G = 0.0, wup = 0.0, rr = 0.0;
           for (int j = U.inc_off[c]; j < U.inc_off[c + 1]; ++j) {
             const int v = U.inc_branch[j];
             const double wv = U.w[v];
@@ -817,16 +828,20 @@ __global__ void __launch_bounds__(kUltraThreads) k_ultra(DevForest F, UltraDev U
               gg += U.gp[v];
               const int kc = U.dn_cls[v];
               if (kc >= 0) {
-                L -= wv * wv / U.Dd[kc];
-                rr += wv * U.r[kc] / U.Dd[kc];
+                const double Dk = U.Dd[kc];
+                G += wv * (U.Gg[kc] / Dk);
+                rr += wv * (U.r[kc] / Dk);
+              } else {
+                G += wv;
               }
             } else {
               gg -= U.gp[v];
+              if (U.up_cls[v] >= 0) wup += wv; else G += wv;
             }
-            L += wv;
           }
           U.g[c] = gg;
-          U.Dd[c] = L;
+          U.Gg[c] = G;
+          U.Dd[c] = G + wup;
           U.r[c] = gg + rr;
         }
         __syncthreads();
@@ -856,7 +871,7 @@ __global__ void __launch_bounds__(kUltraThreads) k_ultra(DevForest F, UltraDev U
       for (int i = nb + threadIdx.x; i < ne; i += kUltraThreads) {
         const int v = F.prob_nodes[i];
         if (!F.free_branch[v]) continue;
-        const double dv = ult_height(U, U.up_cls[v], U.up_off[v], 0.0) - ult_height(U, U.dn_cls[v], U.dn_off[v], 0.0);
+        const double dv = ult_height(U, U.up_cls[v], U.up_off[v]) - ult_height(U, U.dn_cls[v], U.dn_off[v]);
         const double dd = (U.up_cls[v] >= 0 ? U.dt[U.up_cls[v]] : 0.0) - (U.dn_cls[v] >= 0 ? U.dt[U.dn_cls[v]] : 0.0);
         if (dd < 0.0) amax = fmin(amax, 0.99 * (dv - dmin) / (-dd));
         else if (dd > 0.0) amax = fmin(amax, 0.99 * (dmax - dv) / dd);
@@ -864,6 +879,10 @@ __global__ void __launch_bounds__(kUltraThreads) k_ultra(DevForest F, UltraDev U
       lam = block_sum<kUltraThreads>(lam, sh);
       step = -block_min<kUltraThreads>(-step, sh);
       amax = block_min<kUltraThreads>(amax, sh);
+      if (!isfinite(lam) || !isfinite(step)) {
+        status = 2;  // singular Newton system: keep the current heights
+        break;
+      }
       if (!(lam > 0.0) || step * amax <= 1e-15) break;
       double a = amax;
       const double phi0 = ult_phi(F, U, nb, ne, nu0, mu, 0.0, sh);
@@ -893,7 +912,7 @@ __global__ void __launch_bounds__(kUltraThreads) k_ultra(DevForest F, UltraDev U
   for (int i = nb + threadIdx.x; i < ne; i += kUltraThreads) {
     const int v = F.prob_nodes[i];
     if (!F.free_branch[v]) continue;
-    const double dv = ult_height(U, U.up_cls[v], U.up_off[v], 0.0) - ult_height(U, U.dn_cls[v], U.dn_off[v], 0.0);
+    const double dv = ult_height(U, U.up_cls[v], U.up_off[v]) - ult_height(U, U.dn_cls[v], U.dn_off[v]);
     d[v] = dv;
     F.nu_d[F.node_pos[v]] = dv;
   }

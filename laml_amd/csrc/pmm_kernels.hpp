@@ -89,7 +89,8 @@ struct UltraDev {
   const uint8_t* has_free;   // [P]
   double* t;                 // [V] heights (warm start / solution)
   double* dt;                // [V] Newton direction
-  double* Dd;                // [V] eliminated diagonal
+  double* Dd;                // [V] eliminated diagonal (= Gg + weight of the edge to the parent class)
+  double* Gg;                // [V] conductance of the class subtree to the pinned heights
   double* r;                 // [V] eliminated right-hand side
   double* g;                 // [V] gradient
   double* gp;                // [N] branch first derivative
@@ -97,7 +98,7 @@ struct UltraDev {
   double* bA;                // [N] branch objective: -A d + S1 log(1-e^-d) + B log(1-e^-nu d)
   double* bS1;
   double* bB;
-  int32_t* status;           // [P] 0 ok, 1 line search stalled
+  int32_t* status;           // [P] 0 ok, 1 line search stalled, 2 singular Newton system
 };
 
 struct EmState {

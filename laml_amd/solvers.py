@@ -50,6 +50,7 @@ def _device_index():
 
 class EM_solver:
     """GPU-backed replacement of laml_libs.EM_solver.EM_solver."""
+    solver_name = "EM_solver"
 
     def __init__(self, treeList, data, prior, params={'nu': 0, 'phi': 0, 'sigma': 0}):
         # ML_solver.__init__ (ML_solver.py:19-86)
@@ -466,6 +467,7 @@ class EM_solver:
 class fastEM_solver(EM_solver):
     """Surface of laml_libs.fastEM_solver.fastEM_solver (fastEM_solver.py:149-178),
     evaluated by the exact FP64 device engine."""
+    solver_name = "fastEM_solver"
 
     def __init__(self, treeList, data, prior, params={'nu': 0, 'phi': 0, 'sigma': 0}):
         super().__init__(treeList, data, prior, params={'nu': 0, 'phi': 0, 'sigma': 0})

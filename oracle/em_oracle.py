@@ -184,6 +184,40 @@ def resolve_polytomies(root):
     return new_nodes
 
 
+def to_newick(root):
+    """Newick text of an ONode tree (lengths written with repr)."""
+    def rec(v):
+        out = ""
+        if v.children:
+            out = "(" + ",".join(rec(c) for c in v.children) + ")"
+        if v.label is not None:
+            out += str(v.label)
+        if v.edge_length is not None:
+            out += ":" + repr(float(v.edge_length))
+        return out
+    return rec(root) + ";"
+
+
+def find_lca(root, a, b):
+    """Lowest common ancestor of the nodes labelled a and b
+    (laml_libs/lca_lib.py:13-95 answers the same query with an Euler tour
+    and a segment tree; here by climbing from the deeper node)."""
+    by_label = {}
+    depth = {}
+    for v in preorder(root):
+        depth[v] = 0 if v.parent is None else depth[v.parent] + 1
+        if v.label is not None:
+            by_label[v.label] = v
+    x, y = by_label[a], by_label[b]
+    while depth[x] > depth[y]:
+        x = x.parent
+    while depth[y] > depth[x]:
+        y = y.parent
+    while x is not y:
+        x, y = x.parent, y.parent
+    return x
+
+
 def log_sum_exp(xs):
     """laml_libs/EM_solver.py:8-13 (all -inf inputs give NaN, as there)."""
     m = max(xs)
@@ -636,18 +670,32 @@ class OracleEM:
             em_iter += 1
         return -curr, em_iter, status
 
-    def optimize_one(self, randseed, fixed_phi=None, fixed_nu=None):
+    def optimize_one(self, randseed, fixed_phi=None, fixed_nu=None, ultra_constr=False):
         """EM_solver.optimize_one (EM_solver.py:514-527)."""
         rng = _random.Random(randseed)
         x0 = self.ini_all(rng, fixed_phi=fixed_phi, fixed_nu=fixed_nu)
         self.x2params(x0, fixed_phi=fixed_phi, fixed_nu=fixed_nu)
         self.az_partition()
         nllh, it, status = self.em_optimization(optimize_phi=(fixed_phi is None),
-                                                optimize_nu=(fixed_nu is None))
+                                                optimize_nu=(fixed_nu is None), ultra_constr=ultra_constr)
         self.em_iters = it
         return nllh, status
 
-    def optimize(self, initials=1, random_seeds=1221, fixed_phi=None, fixed_nu=None):
+    def mark_fixed(self, fixed_brlen):
+        """Fix branches named by leaf-label anchor pairs at the given lengths
+        (ML_solver.py:287-296); all other marks are cleared."""
+        for t, r in enumerate(self.trees):
+            for v in postorder(r):
+                v.a["fixed"] = False
+            if fixed_brlen is None:
+                continue
+            for (a, b), length in fixed_brlen[t].items():
+                u = find_lca(r, a, b)
+                u.edge_length = length
+                u.a["fixed"] = True
+
+    def optimize(self, initials=1, random_seeds=1221, fixed_phi=None, fixed_nu=None, ultra_constr=False,
+                 fixed_brlen=None):
         """ML_solver.optimize seed handling (ML_solver.py:256-336) for integer
         seeds and a single round (no retries are needed: the oracle M-step
         never fails)."""
@@ -655,7 +703,9 @@ class OracleEM:
         rseeds = [int(rng.random() * 10000) for _ in range(initials)]
         results = []
         for rep in range(initials):
-            nllh, status = self.optimize_one(rseeds[rep], fixed_phi=fixed_phi, fixed_nu=fixed_nu)
+            self.mark_fixed(fixed_brlen)
+            nllh, status = self.optimize_one(rseeds[rep], fixed_phi=fixed_phi, fixed_nu=fixed_nu,
+                                             ultra_constr=ultra_constr)
             results.append((nllh, rep, self.nu, self.phi, self.branch_lengths(), status))
         results.sort(key=lambda t: (t[0], t[1]))
         return results[0]

@@ -75,3 +75,74 @@ def load_example1():
             a, b, c = line.split(",")
             Q[int(a)][int(b)] = float(c)
     return T, msa, Q
+
+
+def load_topology_kats():
+    """Fixture of the reference's topology-search tests: (Q, msa dict by name,
+    brute-force optima, cases)."""
+    with open(os.path.join(GOLDEN, "topology_search_kats.json")) as f:
+        k = json.load(f)
+    Q = [{int(a): b for a, b in q.items()} for q in k["Q"]]
+    return Q, k["msa"], k["brute_force"], k["cases"]
+
+
+def rooted_topologies(labels):
+    """Every rooted binary topology on the labels, as Newick strings, by
+    inserting each further leaf on every edge (root edge included):
+    (2n-3)!! trees."""
+    def insert(leaf, t):
+        out = [(t, leaf)]
+        if isinstance(t, tuple):
+            left, right = t
+            out += [(x, right) for x in insert(leaf, left)] + [(left, x) for x in insert(leaf, right)]
+        return out
+
+    def nwk(t):
+        return t if isinstance(t, str) else "(" + nwk(t[0]) + "," + nwk(t[1]) + ")"
+
+    topos = [(labels[0], labels[1])]
+    for leaf in labels[2:]:
+        topos = [x for t in topos for x in insert(leaf, t)]
+    return [nwk(t) + ";" for t in topos]
+
+
+class OracleSolver:
+    """The oracle behind the solver surface that Topology_search drives
+    (constructor, score_tree, get_tree_newick, get_params), so that the
+    search's host logic can run on the CPU.  score_tree follows
+    ML_solver.optimize(initials=1, random_seeds=None): one seed is drawn from
+    the global generator."""
+    solver_name = "EM_solver"
+
+    def __init__(self, treeList, data, prior, params={'nu': 0, 'phi': 0}):
+        self.treeList = list(treeList)
+        self.charMtrx = data['charMtrx']
+        self.Q = prior['Q']
+        self.params = dict(params)
+        self.trees = list(treeList)
+
+    def score_tree(self, strategy):
+        import random
+        from laml_amd.tree import read_tree_newick
+        from oracle.em_oracle import OracleEM, to_newick
+
+        seed = int(random.random() * 10000)
+        o = OracleEM(self.treeList, self.charMtrx, self.Q)
+        res = o.optimize(initials=1, random_seeds=seed, fixed_phi=strategy['fixed_phi'],
+                         fixed_nu=strategy['fixed_nu'], ultra_constr=strategy['ultra_constr'],
+                         fixed_brlen=strategy.get('fixed_brlen'))
+        nllh, _, nu, phi, _, status = res
+        out = []
+        for r in o.trees:
+            t = read_tree_newick(to_newick(r))
+            t.collapse_short_branches(o.dmin * 0.01)
+            out.append(t.newick())
+        self.trees = out
+        self.params = {'nu': nu, 'phi': phi}
+        return -nllh, status
+
+    def get_tree_newick(self):
+        return list(self.trees)
+
+    def get_params(self):
+        return dict(self.params)
