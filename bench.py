@@ -110,6 +110,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-reps", type=int, default=10)
     ap.add_argument("--batch", type=int, default=16, help="independent problems per step")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the measurement, run 3 plain device copies of this many MiB (a known-size "
+                         "reference for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the PMM path on a single MI355X "
@@ -225,6 +228,13 @@ def main():
     else:
         res["cpu_baseline"] = None
     F.close()
+    if args.calib_copy_mib > 0:
+        src = torch.ones(args.calib_copy_mib * (1 << 20) // 8, dtype=torch.float64, device=0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copies: 3 x %d MiB" % args.calib_copy_mib)
     print(json.dumps(res), flush=True)
 
 

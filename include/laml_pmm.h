@@ -109,6 +109,20 @@ int laml_forest_set_checkpoint(laml_forest* f, int32_t levels);
 /* Rows (node x site) of inside-vector storage currently allocated. */
 int64_t laml_forest_inside_rows(laml_forest* f);
 
+/* Launch shapes of the level-synchronous passes, host arrays of num_levels
+ * entries indexed by level (entry 0 unused).  When the widest site row holds
+ * more than 32 sites, each level runs one wavefront per node over
+ * inside_cpw[h] / outside_cpw[h] consecutive chunks of 64 sites; otherwise
+ * the entries read 0 and are ignored.  top_level is the first level that the
+ * fused top-level kernel runs (num_levels: none); level checkpointing
+ * disables the fused kernel while it is on.  Shapes change the order of
+ * floating-point sums over sites only.
+ * No counterpart in the reference (its passes are Python loops); the values
+ * come from the default heuristic or from laml_amd.autotune. */
+int laml_forest_get_shapes(laml_forest* f, int32_t* inside_cpw, int32_t* outside_cpw, int32_t* top_level);
+int laml_forest_set_shapes(laml_forest* f, const int32_t* inside_cpw, const int32_t* outside_cpw,
+                           int32_t top_level);
+
 /* Per-node az-partition labels, int16 [num_nodes * K] row-major (0 = 'z',
  * -1 = '?', j>0 = local edit j), copied to host.  Replaces reading node.alpha
  * after ML_solver.az_partition (ML_solver.py:125-154). */

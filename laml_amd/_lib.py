@@ -92,6 +92,8 @@ SIGNATURES = {
                                ctypes.c_void_p, ctypes.c_void_p]),
     "laml_forest_set_ultrametric": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(UltraDesc)]),
     "laml_forest_set_heights": (ctypes.c_int, [ctypes.c_void_p, _f64p]),
+    "laml_forest_get_shapes": (ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p]),
+    "laml_forest_set_shapes": (ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, ctypes.c_int32]),
     "laml_forest_ultra_status": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
     "laml_last_error": (ctypes.c_char_p, []),
     "laml_version": (ctypes.c_char_p, []),

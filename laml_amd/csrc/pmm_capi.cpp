@@ -37,9 +37,15 @@ struct laml_forest {
   int h_top = 1 << 30;  // first level handled by the fused top-level kernels
   int h_top_default = 1 << 30;
   int ckpt_L = 0;       // level-checkpoint segment length (0: all inside vectors stored)
+  // per level: 64-site chunks one wavefront of the row kernels walks (W == 64)
+  std::vector<int32_t> in_cpw, out_cpw;
   // host copies for re-layout of the inside vectors
-  std::vector<int32_t> h_parent, h_K;
-  std::vector<int64_t> h_row_off;
+  std::vector<int32_t> h_parent;
+  std::vector<int32_t> h_K;         // padded row stride of each node
+  std::vector<int32_t> h_Kt;        // sites of each node's tree
+  std::vector<int64_t> h_row_off;   // device row offsets (padded)
+  std::vector<int64_t> h_desc_off;  // row offsets of the unpadded description layout
+  int64_t leaf_sites = 0, int_sites = 0;
   void* beta_alloc = nullptr;
   void* bE_alloc = nullptr;
   int64_t beta_rows = 0;
@@ -218,6 +224,8 @@ int validate(const laml_forest_desc* d, Derived& D) {
   }
   const int S = tree_site_base[T];
   if (d->site_qoff[0] != 0) return fail(LAML_EINVAL, "site_qoff[0] != 0");
+  // kernels index the prior table with 32-bit byte offsets
+  if (d->site_qoff[S] >= (1 << 28)) return fail(LAML_EINVAL, "prior table too large (>= 2^28 entries)");
   for (int s = 0; s < S; ++s)
     if (d->site_qoff[s + 1] < d->site_qoff[s]) return fail(LAML_EINVAL, "site_qoff not monotone");
   for (int v = 0; v < n; ++v) {
@@ -286,13 +294,35 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   Derived D;
   int rc = validate(d, D);
   if (rc) return rc;
-  std::vector<int64_t>& row_off = D.row_off;
   std::vector<int32_t>& prob_tree_off = D.prob_tree_off;
   std::vector<int32_t>& prob_node_off = D.prob_node_off;
   std::vector<int32_t>& prob_nodes = D.prob_nodes;
   std::vector<int32_t>& tree_site_base = D.tree_site_base;
-  const int64_t leaf_rows = D.leaf_rows, int_rows = D.int_rows;
   const int max_k = D.max_k;
+  // Device rows are padded to a multiple of the segment width, so that the
+  // slice of a row one wavefront touches starts on a cache-line boundary
+  // (64 sites = 128 B of int16, 1 KiB of double2).
+  const int Wd = laml::seg_width(max_k);
+  const int Nn = d->num_nodes, nl = d->num_leaves;
+  std::vector<int64_t> row_off(Nn, 0);
+  std::vector<int32_t> stride(Nn, 0);
+  int64_t leaf_rows = 0, int_rows = 0;
+  for (int v = 0; v < Nn; ++v) {
+    const int K = d->tree_nsites[d->tree_of[v]];
+    stride[v] = (K + Wd - 1) / Wd * Wd;
+    if (v < nl) {
+      row_off[v] = leaf_rows;
+      leaf_rows += stride[v];
+    } else {
+      row_off[v] = int_rows;
+      int_rows += stride[v];
+    }
+  }
+  std::vector<int16_t> codes_p((size_t)std::max<int64_t>(leaf_rows, 1), 0);
+  for (int v = 0; v < nl; ++v) {
+    const int K = d->tree_nsites[d->tree_of[v]];
+    std::memcpy(codes_p.data() + row_off[v], d->leaf_codes + D.row_off[v], sizeof(int16_t) * K);
+  }
   int ndev = 0;
   LAML_HIP(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(LAML_EINVAL, "device out of range");
@@ -325,6 +355,8 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   F.dmax = d->dmax;
   F.chunk_w = f->W;
   F.max_chunks = (max_k + f->W - 1) / f->W;
+  f->in_cpw = laml::default_cpw(f->level_off, f->W, F.max_chunks);
+  f->out_cpw = f->in_cpw;
   std::vector<int32_t> tree_prob(d->tree_problem, d->tree_problem + T);
 #define UP(dst, src, n) \
   if ((rc = f->upload(&F.dst, src, (size_t)(n)))) { delete f; return rc; }
@@ -352,7 +384,7 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   UP(prob_nobs_phi, D.prob_nobs_phi.data(), P);
   UP(site_qoff, d->site_qoff, S + 1);
   UP(prior, d->prior, d->site_qoff[S]);
-  UP(codes, d->leaf_codes, leaf_rows);
+  UP(codes, codes_p.data(), leaf_rows);
 #undef UP
 #define AL(dst, type, n) \
   if ((rc = f->alloc<type>(&dst, (size_t)(n)))) { delete f; return rc; }
@@ -406,8 +438,12 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   F.brow = F.row_off;
   f->h_parent.assign(d->parent, d->parent + N);
   f->h_row_off = row_off;
-  f->h_K.resize(N);
-  for (int v = 0; v < N; ++v) f->h_K[v] = d->tree_nsites[d->tree_of[v]];
+  f->h_K = stride;
+  f->h_desc_off = D.row_off;
+  f->leaf_sites = D.leaf_rows;
+  f->int_sites = D.int_rows;
+  f->h_Kt.resize(N);
+  for (int v = 0; v < N; ++v) f->h_Kt[v] = d->tree_nsites[d->tree_of[v]];
   F.coef = coef;
   F.beta = beta;
   F.bE = bE;
@@ -464,16 +500,24 @@ int laml_forest_set_flags(laml_forest* f, const uint32_t* flags_host) {
 int laml_forest_labels(laml_forest* f, int16_t* labels_host, int64_t capacity) {
   if (!f || !labels_host) return fail(LAML_EINVAL, "null argument");
   DeviceGuard guard(f->device);
-  const int64_t need = f->leaf_rows + f->int_rows;
+  const int64_t need = f->leaf_sites + f->int_sites;
   if (capacity < need) return fail(LAML_EINVAL, "labels capacity too small");
   std::vector<int16_t> codes(f->leaf_rows), lab(f->int_rows);
   LAML_HIP(hipMemcpy(codes.data(), f->F.codes, sizeof(int16_t) * f->leaf_rows, hipMemcpyDeviceToHost));
   LAML_HIP(hipMemcpy(lab.data(), f->F.labels, sizeof(int16_t) * f->int_rows, hipMemcpyDeviceToHost));
-  for (int64_t i = 0; i < f->leaf_rows; ++i) {
-    const int c = codes[i];
-    labels_host[i] = (int16_t)(c > 0 ? c : (c == 0 ? 0 : -1));
+  const int N = f->F.num_nodes, n = f->F.num_leaves;
+  for (int v = 0; v < N; ++v) {
+    const int K = f->h_Kt[v];
+    int16_t* out = labels_host + (v < n ? 0 : f->leaf_sites) + f->h_desc_off[v];
+    if (v < n) {
+      for (int k = 0; k < K; ++k) {
+        const int c = codes[f->h_row_off[v] + k];
+        out[k] = (int16_t)(c > 0 ? c : (c == 0 ? 0 : -1));
+      }
+    } else {
+      std::memcpy(out, lab.data() + f->h_row_off[v], sizeof(int16_t) * K);
+    }
   }
-  std::memcpy(labels_host + f->leaf_rows, lab.data(), sizeof(int16_t) * f->int_rows);
   return LAML_OK;
 }
 
@@ -549,6 +593,37 @@ int laml_forest_set_checkpoint(laml_forest* f, int32_t levels) {
 }
 
 int64_t laml_forest_inside_rows(laml_forest* f) { return f ? f->beta_rows : -1; }
+
+int laml_forest_get_shapes(laml_forest* f, int32_t* inside_cpw, int32_t* outside_cpw, int32_t* top_level) {
+  if (!f) return fail(LAML_EINVAL, "null forest");
+  const int H = (int)f->level_off.size() - 1;
+  for (int h = 0; h < H; ++h) {
+    if (inside_cpw) inside_cpw[h] = f->in_cpw.empty() ? 0 : f->in_cpw[h];
+    if (outside_cpw) outside_cpw[h] = f->out_cpw.empty() ? 0 : f->out_cpw[h];
+  }
+  if (top_level) *top_level = f->h_top_default;
+  return LAML_OK;
+}
+
+int laml_forest_set_shapes(laml_forest* f, const int32_t* inside_cpw, const int32_t* outside_cpw, int32_t top_level) {
+  if (!f) return fail(LAML_EINVAL, "null forest");
+  const int H = (int)f->level_off.size() - 1;
+  if (top_level < 1 || top_level > H) return fail(LAML_EINVAL, "top_level out of range [1, num_levels]");
+  if (f->W == 64) {
+    for (int h = 1; h < H; ++h) {
+      if ((inside_cpw && (inside_cpw[h] < 1 || inside_cpw[h] > f->F.max_chunks)) ||
+          (outside_cpw && (outside_cpw[h] < 1 || outside_cpw[h] > f->F.max_chunks)))
+        return fail(LAML_EINVAL, "chunks per wavefront out of range [1, max_chunks]");
+    }
+    for (int h = 1; h < H; ++h) {
+      if (inside_cpw) f->in_cpw[h] = inside_cpw[h];
+      if (outside_cpw) f->out_cpw[h] = outside_cpw[h];
+    }
+  }
+  f->h_top_default = top_level;
+  if (f->ckpt_L == 0) f->h_top = top_level;
+  return LAML_OK;
+}
 
 int laml_forest_set_ultrametric(laml_forest* f, const laml_ultra_desc* u) {
   if (!f) return fail(LAML_EINVAL, "null forest");
@@ -651,7 +726,7 @@ int laml_nll(laml_forest* f, const double* d, const double* nu, const double* ph
   DevForest F = f->F;
   F.phi = phi;
   laml::run_coef(F, d, nu, nullptr, s);
-  laml::run_inside(F, f->level_off, f->W, f->h_top, nullptr, s);
+  laml::run_inside(F, f->level_off, f->W, f->h_top, f->in_cpw, nullptr, s);
   laml::run_root(F, f->W, site_llh, nullptr, s);
   EmState none{};
   laml::run_llh(F, nll, none, 0, 0, 0.0, s);
@@ -666,11 +741,11 @@ int laml_estep(laml_forest* f, const double* d, const double* nu, const double* 
   DevForest F = f->F;
   F.phi = phi;
   laml::run_coef(F, d, nu, nullptr, s);
-  laml::run_inside(F, f->level_off, f->W, f->h_top, nullptr, s);
+  laml::run_inside(F, f->level_off, f->W, f->h_top, f->in_cpw, nullptr, s);
   laml::run_root(F, f->W, nullptr, nullptr, s);
   EmState none{};
   laml::run_llh(F, nll, none, 0, 0, 0.0, s);
-  (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, nullptr, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, nullptr, s));
+  (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, f->in_cpw, f->out_cpw, nullptr, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, f->out_cpw, nullptr, s));
   laml::run_counts(F, counts, s);
   laml::run_rtilde(F, rtilde, nullptr, s);
   return check_launch();
@@ -683,7 +758,7 @@ int laml_outside(laml_forest* f, const double* phi, double* counts, double* rtil
   DevForest F = f->F;
   F.phi = phi;
   laml::run_root(F, f->W, nullptr, nullptr, s);
-  (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, nullptr, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, nullptr, s));
+  (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, f->in_cpw, f->out_cpw, nullptr, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, f->out_cpw, nullptr, s));
   laml::run_counts(F, counts, s);
   laml::run_rtilde(F, rtilde, nullptr, s);
   return check_launch();
@@ -697,7 +772,7 @@ int laml_grad(laml_forest* f, const double* d, const double* nu, const double* p
   DevForest F = f->F;
   F.phi = phi;
   laml::run_root(F, f->W, nullptr, nullptr, s);
-  (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, nullptr, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, nullptr, s));
+  (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, f->in_cpw, f->out_cpw, nullptr, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, f->out_cpw, nullptr, s));
   laml::run_grad(F, d, nu, phi, grad_d, grad_nu, grad_phi, s);
   return check_launch();
 }
@@ -728,11 +803,11 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
   const int chunk = 8;
   for (int step = 0; step <= max_iter; ++step) {
     laml::run_coef(F, d, nu, em.done, s);
-    laml::run_inside(F, f->level_off, f->W, f->h_top, em.done, s);
+    laml::run_inside(F, f->level_off, f->W, f->h_top, f->in_cpw, em.done, s);
     laml::run_root(F, f->W, nullptr, em.done, s);
     laml::run_llh(F, nll, em, step, max_iter, tol, s);
     if (step < max_iter) {
-      (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, em.done, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, em.done, s));
+      (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, f->in_cpw, f->out_cpw, em.done, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, f->out_cpw, em.done, s));
       laml::run_mstep(F, nullptr, nullptr, d, nu, phi, em.done, s, f->ultra_on ? &f->U : nullptr);
     }
     int rc = check_launch();

@@ -85,36 +85,125 @@ __device__ __forceinline__ void load_child(const DevForest& F, int c, bool leaf,
   }
 }
 
-// inside vector (below the node) of internal node v at site k
-__device__ __forceinline__ void inside_node(const DevForest& F, int v, int k, const int32_t* __restrict__ done) {
-  const int t = F.tree_of[v];
-  const int p = F.tree_prob[t];
-  if (k >= F.tree_nsites[t]) return;
-  if (done && done[p]) return;
-  const int sb = F.tree_site_base[t];
-  const int c1 = F.child0[v], c2 = F.child1[v];
-  const Coef cf1 = F.coef[c1], cf2 = F.coef[c2];
-  const double phi = F.phi[p];
-  const int64_t rv = F.row_off[v];
-  const int lv = F.labels[rv + k];
+// Element k of a row: the byte offset is formed in 32 bits, so that with a
+// wave-uniform row pointer the access is a scalar base plus a 32-bit lane
+// offset (rows hold fewer than 2^24 sites; the forest checks it).
+template <typename T>
+__device__ __forceinline__ T& row_at(T* p, unsigned k) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + k * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ const T& row_at(const T* p, unsigned k) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + k * (unsigned)sizeof(T));
+}
+
+// Rows of one child node at its parent's sites: leaf codes, or label +
+// inside vector.  Held as pointers to the row start so that, where the node is
+// wave-uniform, every per-site access is a scalar base plus a 32-bit lane
+// offset.
+struct ChildRows {
+  const int16_t* code;   // leaf codes (leaf) or az-partition labels (internal)
+  const double2* beta;   // inside mantissas (internal)
+  const int32_t* bE;     // inside exponent (internal)
+};
+
+__device__ __forceinline__ ChildRows child_rows(const DevForest& F, int c, bool leaf) {
+  ChildRows r;
+  const int64_t ro = F.row_off[c];
+  if (leaf) {
+    r.code = F.codes + ro;
+    r.beta = nullptr;
+    r.bE = nullptr;
+  } else {
+    const int64_t br = F.brow[c];
+    r.code = F.labels + ro;
+    r.beta = F.beta + br;
+    r.bE = F.bE + br;
+  }
+  return r;
+}
+
+__device__ __forceinline__ void load_child_rows(const ChildRows& r, bool leaf, int k, double phi, int& lab,
+                                                double& b0, double& bA, int& E, int& code) {
+  if (leaf) {
+    code = row_at(r.code, k);
+    leaf_beta(code, phi, lab, b0, bA);
+    E = 0;
+  } else {
+    code = 0;
+    lab = row_at(r.code, k);
+    const double2 bb = row_at(r.beta, k);
+    b0 = bb.x;
+    bA = bb.y;
+    E = row_at(r.bE, k);
+  }
+}
+
+// per-node values of the inside pass (uniform over the sites of the node)
+struct InNode {
+  int K;
+  bool lf[2];
+  Coef cf[2];
+  ChildRows c[2];
+  const int16_t* lab;    // the node's labels
+  double2* beta;         // the node's inside row
+  int32_t* bE;
+  const int32_t* qoff;   // prior offsets of the tree's sites
+  double phi;
+};
+
+__device__ __forceinline__ void in_node_load(const DevForest& F, int v, int t, int p, int K, InNode& n) {
+  n.K = K;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int c = i == 0 ? F.child0[v] : F.child1[v];
+    n.lf[i] = c < F.num_leaves;
+    n.cf[i] = F.coef[c];
+    n.c[i] = child_rows(F, c, n.lf[i]);
+  }
+  n.lab = F.labels + F.row_off[v];
+  const int64_t bv = F.brow[v];
+  n.beta = F.beta + bv;
+  n.bE = F.bE + bv;
+  n.qoff = F.site_qoff + F.tree_site_base[t];
+  n.phi = F.phi[p];
+}
+
+// inside vector (below the node) of internal node n at site k.  LEAF0 /
+// LEAF1: 1 = that child is a leaf, 0 = internal, -1 = read n.lf at run time.
+template <int LEAF0, int LEAF1>
+__device__ __forceinline__ void inside_site(const DevForest& F, const InNode& n, int k) {
+  const bool lf[2] = {LEAF0 >= 0 ? LEAF0 == 1 : n.lf[0], LEAF1 >= 0 ? LEAF1 == 1 : n.lf[1]};
+  const int lv = row_at(n.lab, k);
   int la, lb, Ea, Eb, ca, cb;
   double a0, aA, b0, bA;
-  load_child(F, c1, c1 < F.num_leaves, k, phi, la, a0, aA, Ea, ca);
-  load_child(F, c2, c2 < F.num_leaves, k, phi, lb, b0, bA, Eb, cb);
-  const int qo = F.site_qoff[sb + k] - 1;
-  const double qa = la > 0 ? F.prior[qo + la] : 1.0;
-  const double qb = lb > 0 ? F.prior[qo + lb] : 1.0;
-  const Eps ea = edge(la, a0, aA, Ea, qa, cf1);
-  const Eps eb = edge(lb, b0, bA, Eb, qb, cf2);
+  load_child_rows(n.c[0], lf[0], k, n.phi, la, a0, aA, Ea, ca);
+  load_child_rows(n.c[1], lf[1], k, n.phi, lb, b0, bA, Eb, cb);
+  const int qo = row_at(n.qoff, k) - 1;
+  const double qa = la > 0 ? row_at(F.prior, (unsigned)(qo + la)) : 1.0;
+  const double qb = lb > 0 ? row_at(F.prior, (unsigned)(qo + lb)) : 1.0;
+  const Eps ea = edge(la, a0, aA, Ea, qa, n.cf[0]);
+  const Eps eb = edge(lb, b0, bA, Eb, qb, n.cf[1]);
   const double B0 = ea.e0 * eb.e0;
   const double BA = lv != 0 ? ea.eA * eb.eA : 0.0;
   // renormalise the pair by a power of two (bounded so that a denormal pair
   // cannot produce an infinite scale)
   const int sh = min(-exp_of(fmax(B0, BA)), 1000);
   const double sc = ldexp(1.0, sh);
-  const int64_t bv = F.brow[v];
-  F.beta[bv + k] = make_double2(B0 * sc, BA * sc);
-  F.bE[bv + k] = ea.e + eb.e - sh;
+  row_at(n.beta, k) = make_double2(B0 * sc, BA * sc);
+  row_at(n.bE, k) = ea.e + eb.e - sh;
+}
+
+// segment form: one site k of node v
+__device__ __forceinline__ void inside_node(const DevForest& F, int v, int k, const int32_t* __restrict__ done) {
+  const int t = F.tree_of[v];
+  const int p = F.tree_prob[t];
+  const int K = F.tree_nsites[t];
+  if (k >= K) return;
+  if (done && done[p]) return;
+  InNode n;
+  in_node_load(F, v, t, p, K, n);
+  inside_site<-1, -1>(F, n, k);
 }
 
 template <int W>
@@ -122,6 +211,42 @@ __global__ void __launch_bounds__(256) k_inside(DevForest F, int beg, int cnt, c
   Seg<W> g;
   if (!g.init(beg, cnt)) return;
   inside_node(F, g.v, g.k, done);
+}
+
+// Row form (K > 32): one wavefront per node walks chunks [c0, c0 + cpw) of 64
+// sites.  The node number is wave-uniform, so the node's metadata are scalar
+// loads and the leaf tests scalar branches.
+template <int LEAF0, int LEAF1>
+__device__ __forceinline__ void inside_row_run(const DevForest& F, const InNode& n, int c0, int c1, int lane) {
+  for (int c = c0; c < c1; ++c) {
+    const int k = c * 64 + lane;
+    if (k < n.K) inside_site<LEAF0, LEAF1>(F, n, k);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_inside_rows(DevForest F, int beg, int cnt, int cpw,
+                                                      const int32_t* __restrict__ done) {
+  const int wv = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (wv >= cnt) return;
+  const int lane = threadIdx.x & 63;
+  const int v = beg + wv;
+  const int t = F.tree_of[v];
+  const int p = F.tree_prob[t];
+  const int K = F.tree_nsites[t];
+  const int nch = (K + 63) >> 6;
+  const int c0 = (int)blockIdx.y * cpw;
+  if (c0 >= nch) return;
+  if (done && done[p]) return;
+  const int c1 = min(c0 + cpw, nch);
+  InNode n;
+  in_node_load(F, v, t, p, K, n);
+  if (n.lf[0]) {
+    if (n.lf[1]) inside_row_run<1, 1>(F, n, c0, c1, lane);
+    else inside_row_run<1, 0>(F, n, c0, c1, lane);
+  } else {
+    if (n.lf[1]) inside_row_run<0, 1>(F, n, c0, c1, lane);
+    else inside_row_run<0, 0>(F, n, c0, c1, lane);
+  }
 }
 
 // Inside pass over the top levels [h0, h1): one block per site chunk walks
@@ -206,9 +331,159 @@ __global__ void __launch_bounds__(256) k_root(DevForest F, double* __restrict__ 
 
 // Outside pass for the children of the internal nodes of one level, with the
 // posterior expected transition counts of both child branches and the
-// dropout responsibilities of missing leaf children (one partial per site
-// chunk).
-// All lanes of the W-lane segment call this together (segment shuffles).
+// dropout responsibilities of missing leaf children.
+
+// per-node values of the outside pass (uniform over the sites of the node)
+struct OutNode {
+  int K;
+  int ch[2];
+  bool lf[2];
+  Coef cf[2];
+  ChildRows c[2];
+  double4* ca[2];        // children's outside rows (internal children)
+  int32_t* caE[2];
+  int16_t* caG[2];
+  const double4* a;      // the node's outside row
+  const int32_t* aE;
+  const int16_t* aG;
+  const double* Lm;      // the tree's site likelihoods (mantissa, exponent)
+  const int32_t* LE;
+  const int32_t* qoff;
+  double phi;
+};
+
+__device__ __forceinline__ void out_node_load(const DevForest& F, int u, int t, int p, int K, OutNode& n) {
+  n.K = K;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int c = i == 0 ? F.child0[u] : F.child1[u];
+    n.ch[i] = c;
+    n.lf[i] = c < F.num_leaves;
+    n.cf[i] = F.coef[c];
+    n.c[i] = child_rows(F, c, n.lf[i]);
+    const int64_t rc = F.row_off[c];
+    n.ca[i] = F.alpha + rc;
+    n.caE[i] = F.aE + rc;
+    n.caG[i] = F.aG + rc;
+  }
+  const int64_t ru = F.row_off[u];
+  n.a = F.alpha + ru;
+  n.aE = F.aE + ru;
+  n.aG = F.aG + ru;
+  const int sb = F.tree_site_base[t];
+  n.Lm = F.Lm + sb;
+  n.LE = F.LE + sb;
+  n.qoff = F.site_qoff + sb;
+  n.phi = F.phi[p];
+}
+
+// One site k of node n: the children's outside states and the posterior
+// counts of both child branches, added to acc; the dropout responsibility of
+// missing leaf children added to rt.  LEAF0 / LEAF1: 1 = that child is a leaf,
+// 0 = internal, -1 = read n.lf at run time.
+template <int LEAF0, int LEAF1>
+__device__ __forceinline__ void outside_site(const DevForest& F, const OutNode& n, int k, double (&acc)[2][5],
+                                             double& rt) {
+  const bool lf[2] = {LEAF0 >= 0 ? LEAF0 == 1 : n.lf[0], LEAF1 >= 0 ? LEAF1 == 1 : n.lf[1]};
+  const double4 au = row_at(n.a, k);
+  const int Eo = row_at(n.aE, k);
+  const int gu = row_at(n.aG, k);
+  const double iL = 1.0 / row_at(n.Lm, k);
+  const int LE = row_at(n.LE, k);
+  const int qo = row_at(n.qoff, k) - 1;
+  int lab[2], E[2], code[2];
+  double b0[2], bA[2], q[2];
+  Eps ep[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    load_child_rows(n.c[i], lf[i], k, n.phi, lab[i], b0[i], bA[i], E[i], code[i]);
+    q[i] = lab[i] > 0 ? row_at(F.prior, (unsigned)(qo + lab[i])) : 1.0;
+    ep[i] = edge(lab[i], b0[i], bA[i], E[i], q[i], n.cf[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int w = 1 - i;
+    const int lw = lab[w];
+    const Coef& cf = n.cf[i];
+    // values below share the exponent ea = Eo + e_w:
+    // A = alpha_u(0) eps_w(0) (joint of the data outside v with u = 0);
+    // G(a) = alpha_u(a) eps_w(a) = q_a Gy + [a = gg] Gz
+    const double A = au.x * ep[w].e0;
+    const double Gy = lw < 0 ? au.z * ep[w].eA : 0.0;
+    const double Gz = lw < 0 ? au.w * ep[w].eA
+                             : (lw > 0 ? fma(q[w], au.z, lw == gu ? au.w : 0.0) * ep[w].eA : 0.0);
+    const int gg = lw < 0 ? gu : (lw > 0 ? lw : 0);
+    const int ea = Eo + ep[w].e;
+    const double sumG = Gy + Gz;
+    const double av0 = A * cf.p1n;
+    const double base = A * cf.mut;
+    const double Yv = fma(cf.pn, Gy, base);
+    const double Zv = cf.pn * Gz;
+    if (!lf[i]) {
+      // alpha_v(-1) = GS 2^Eo + sil (A + sumG) 2^ea, GS = alpha_u(-1) eps_w(-1).
+      // The stored exponent comes from the largest value of either term
+      // group (GS is often zero), so the quad is normalised in one step.
+      const double GS = lw < 0 ? au.y : 0.0;
+      const double T = ldexp(cf.msil * (A + sumG), cf.esil);
+      const double mx = fmax(fmax(av0, T), fmax(Yv, Zv));
+      const int ta = exp_of(mx);
+      const int tg = GS != 0.0 ? Eo + exp_of(GS) : kZeroTop;
+      int ec = max(mx != 0.0 ? ea + ta : kZeroTop, tg);
+      if (ec == kZeroTop) ec = Eo;
+      const double sa = ldexp(1.0, min(ea - ec, 1000));
+      const double sg = ldexp(1.0, min(Eo - ec, 1000));
+      row_at(n.ca[i], k) = make_double4(av0 * sa, fma(GS, sg, T * sa), Yv * sa, Zv * sa);
+      row_at(n.caE[i], k) = ec;
+      row_at(n.caG[i], k) = (int16_t)gg;
+    }
+    const int lv = lab[i];
+    if (lv == 0) {
+      acc[i][0] += 1.0;
+    } else {
+      // S0, S1, S3 carry 2^(ea + E_v) / L; S2, S4 carry sil 2^ea / L
+      const double f1 = ldexp(iL, min(ea + E[i] - LE, 1000));
+      acc[i][0] = fma(av0 * b0[i], f1, acc[i][0]);
+      acc[i][1] = fma(base * (q[i] * bA[i]), f1, acc[i][1]);
+      const double t3 = lv > 0 ? fma(q[i], Gy, lv == gg ? Gz : 0.0) : sumG;
+      acc[i][3] = fma(t3 * (cf.pn * bA[i]), f1, acc[i][3]);
+      if (lv < 0) {
+        const double f2 = ldexp(iL * cf.msil, min(ea + cf.esil - LE, 1000));
+        acc[i][2] = fma(A, f2, acc[i][2]);
+        acc[i][4] = fma(sumG, f2, acc[i][4]);
+        if (lf[i] && code[i] == -1) rt = fma(n.phi * (av0 + Yv + Zv), ldexp(iL, min(ea - LE, 1000)), rt);
+      }
+    }
+  }
+}
+
+// segment sums of the partial counts; the last lane of the segment writes
+// them as partial number `slot` of each child branch (and of the node's
+// dropout sum), then clears partials slot+1 .. slot_end-1
+template <int W>
+__device__ __forceinline__ void out_write_partials(const DevForest& F, int u, const int (&ch)[2], int lane, int slot,
+                                                   int slot_end, double (&acc)[2][5], double rt) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[i][j] = seg_sum_last<W>(acc[i][j]);
+  rt = seg_sum_last<W>(rt);
+  if (lane == W - 1) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      double* c = F.cpart + ((int64_t)ch[i] * F.max_chunks + slot) * 5;
+#pragma unroll
+      for (int j = 0; j < 5; ++j) c[j] = acc[i][j];
+      for (int s2 = slot + 1; s2 < slot_end; ++s2)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) F.cpart[((int64_t)ch[i] * F.max_chunks + s2) * 5 + j] = 0.0;
+    }
+    F.rt_part[(int64_t)u * F.max_chunks + slot] = rt;
+    for (int s2 = slot + 1; s2 < slot_end; ++s2) F.rt_part[(int64_t)u * F.max_chunks + s2] = 0.0;
+  }
+}
+
+// Segment form: the W lanes of a segment handle sites chunk*W + lane of node
+// u (all lanes of the segment call this together).
 template <int W>
 __device__ __forceinline__ void outside_node(const DevForest& F, int u, int chunk, int lane,
                                              const int32_t* __restrict__ done) {
@@ -218,103 +493,60 @@ __device__ __forceinline__ void outside_node(const DevForest& F, int u, int chun
   const int K = F.tree_nsites[t];
   if (chunk * W >= K) return;
   if (done && done[p]) return;
-  const bool act = k < K;
-  const int ch[2] = {F.child0[u], F.child1[u]};
-  const bool lf[2] = {ch[0] < F.num_leaves, ch[1] < F.num_leaves};
+  OutNode n;
+  out_node_load(F, u, t, p, K, n);
   double acc[2][5];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 5; ++j) acc[i][j] = 0.0;
   double rt = 0.0;
-  if (act) {
-    const int sb = F.tree_site_base[t];
-    const Coef cf[2] = {F.coef[ch[0]], F.coef[ch[1]]};
-    const int64_t rc[2] = {F.row_off[ch[0]], F.row_off[ch[1]]};
-    const int64_t ru = F.row_off[u];
-    const double phi = F.phi[p];
-    const double4 au = F.alpha[ru + k];
-    const int Eo = F.aE[ru + k];
-    const int gu = F.aG[ru + k];
-    const double iL = 1.0 / F.Lm[sb + k];
-    const int LE = F.LE[sb + k];
-    const int qo = F.site_qoff[sb + k] - 1;
-    int lab[2], E[2], code[2];
-    double b0[2], bA[2], q[2];
-    Eps ep[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      load_child(F, ch[i], lf[i], k, phi, lab[i], b0[i], bA[i], E[i], code[i]);
-      q[i] = lab[i] > 0 ? F.prior[qo + lab[i]] : 1.0;
-      ep[i] = edge(lab[i], b0[i], bA[i], E[i], q[i], cf[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int w = 1 - i;
-      const int lw = lab[w];
-      // values below share the exponent ea = Eo + e_w:
-      // A = alpha_u(0) eps_w(0) (joint of the data outside v with u = 0);
-      // G(a) = alpha_u(a) eps_w(a) = q_a Gy + [a = gg] Gz
-      const double A = au.x * ep[w].e0;
-      const double Gy = lw < 0 ? au.z * ep[w].eA : 0.0;
-      const double Gz = lw < 0 ? au.w * ep[w].eA
-                               : (lw > 0 ? fma(q[w], au.z, lw == gu ? au.w : 0.0) * ep[w].eA : 0.0);
-      const int gg = lw < 0 ? gu : (lw > 0 ? lw : 0);
-      const int ea = Eo + ep[w].e;
-      const double sumG = Gy + Gz;
-      const double av0 = A * cf[i].p1n;
-      const double base = A * cf[i].mut;
-      const double Yv = fma(cf[i].pn, Gy, base);
-      const double Zv = cf[i].pn * Gz;
-      if (!lf[i]) {
-        // alpha_v(-1) = GS 2^Eo + sil (A + sumG) 2^ea, GS = alpha_u(-1) eps_w(-1).
-        // The stored exponent comes from the largest value of either term
-        // group (GS is often zero), so the quad is normalised in one step.
-        const double GS = lw < 0 ? au.y : 0.0;
-        const double T = ldexp(cf[i].msil * (A + sumG), cf[i].esil);
-        const int ta = exp_of(fmax(fmax(av0, T), fmax(Yv, Zv)));
-        const int tg = GS != 0.0 ? Eo + exp_of(GS) : kZeroTop;
-        const bool anya = fmax(fmax(av0, T), fmax(Yv, Zv)) != 0.0;
-        int ec = max(anya ? ea + ta : kZeroTop, tg);
-        if (ec == kZeroTop) ec = Eo;
-        const double sa = ldexp(1.0, min(ea - ec, 1000));
-        const double sg = ldexp(1.0, min(Eo - ec, 1000));
-        F.alpha[rc[i] + k] = make_double4(av0 * sa, fma(GS, sg, T * sa), Yv * sa, Zv * sa);
-        F.aE[rc[i] + k] = ec;
-        F.aG[rc[i] + k] = (int16_t)gg;
-      }
-      const int lv = lab[i];
-      if (lv == 0) {
-        acc[i][0] = 1.0;
-      } else {
-        // S0, S1, S3 carry 2^(ea + E_v) / L; S2, S4 carry sil 2^ea / L
-        const double f1 = ldexp(iL, min(ea + E[i] - LE, 1000));
-        acc[i][0] = av0 * b0[i] * f1;
-        acc[i][1] = base * (q[i] * bA[i]) * f1;
-        const double t3 = lv > 0 ? fma(q[i], Gy, lv == gg ? Gz : 0.0) : sumG;
-        acc[i][3] = t3 * (cf[i].pn * bA[i]) * f1;
-        if (lv < 0) {
-          const double f2 = ldexp(iL * cf[i].msil, min(ea + cf[i].esil - LE, 1000));
-          acc[i][2] = A * f2;
-          acc[i][4] = sumG * f2;
-          if (lf[i] && code[i] == -1) rt += phi * (av0 + Yv + Zv) * ldexp(iL, min(ea - LE, 1000));
-        }
-      }
-    }
-  }
+  if (k < K) outside_site<-1, -1>(F, n, k, acc, rt);
+  out_write_partials<W>(F, u, n.ch, lane, chunk, chunk + 1, acc, rt);
+}
+
+// Row form (K > 32): one wavefront per node, so the node number is
+// wave-uniform (its metadata are scalar loads and the leaf tests scalar
+// branches); the wave walks chunks [c0, c0 + cpw) of 64 sites, accumulating
+// per lane, and reduces once.
+template <int LEAF0, int LEAF1>
+__device__ __forceinline__ void outside_row_run(const DevForest& F, int u, const OutNode& n, int c0, int c1,
+                                                int lane) {
+  double acc[2][5];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 5; ++j) acc[i][j] = seg_sum_last<W>(acc[i][j]);
-  rt = seg_sum_last<W>(rt);
-  if (lane == W - 1) {
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      double* c = F.cpart + ((int64_t)ch[i] * F.max_chunks + chunk) * 5;
-#pragma unroll
-      for (int j = 0; j < 5; ++j) c[j] = acc[i][j];
-    }
-    F.rt_part[(int64_t)u * F.max_chunks + chunk] = rt;
+    for (int j = 0; j < 5; ++j) acc[i][j] = 0.0;
+  double rt = 0.0;
+  for (int c = c0; c < c1; ++c) {
+    const int k = c * 64 + lane;
+    if (k < n.K) outside_site<LEAF0, LEAF1>(F, n, k, acc, rt);
+  }
+  out_write_partials<64>(F, u, n.ch, lane, c0, c1, acc, rt);
+}
+
+__global__ void __launch_bounds__(256) k_outside_rows(DevForest F, int beg, int cnt, int cpw,
+                                                       const int32_t* __restrict__ done) {
+  const int wv = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (wv >= cnt) return;
+  const int lane = threadIdx.x & 63;
+  const int u = beg + wv;
+  const int t = F.tree_of[u];
+  const int p = F.tree_prob[t];
+  const int K = F.tree_nsites[t];
+  const int nch = (K + 63) >> 6;
+  const int c0 = (int)blockIdx.y * cpw;
+  if (c0 >= nch) return;
+  if (done && done[p]) return;
+  const int c1 = min(c0 + cpw, nch);
+  OutNode n;
+  out_node_load(F, u, t, p, K, n);
+  if (n.lf[0]) {
+    if (n.lf[1]) outside_row_run<1, 1>(F, u, n, c0, c1, lane);
+    else outside_row_run<1, 0>(F, u, n, c0, c1, lane);
+  } else {
+    if (n.lf[1]) outside_row_run<0, 1>(F, u, n, c0, c1, lane);
+    else outside_row_run<0, 0>(F, u, n, c0, c1, lane);
   }
 }
 
@@ -957,17 +1189,58 @@ void run_coef(const DevForest& F, const double* d, const double* nu, const int32
   hipLaunchKernelGGL(k_coef, dim3(grid), dim3(256), 0, s, F, d, nu, done);
 }
 
-void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const int32_t* done,
-                hipStream_t s) {
+// one level of the inside pass: row kernels when a per-level shape is given
+// (W == 64), segment kernels otherwise
+static void inside_level(const DevForest& F, const std::vector<int32_t>& lvl, int W, const std::vector<int32_t>& cpw,
+                         int h, const int32_t* done, hipStream_t s) {
+  const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
+  if (cnt <= 0) return;
+  if (W == 64 && !cpw.empty() && cpw[h] > 0) {
+    const int c = cpw[h];
+    hipLaunchKernelGGL(k_inside_rows, dim3((cnt + 3) / 4, (F.max_chunks + c - 1) / c), dim3(256), 0, s, F, beg, cnt,
+                       c, done);
+    return;
+  }
+  LAML_DISPATCH_W(W, {
+    hipLaunchKernelGGL(k_inside<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt, done);
+  });
+}
+
+static void outside_level(const DevForest& F, const std::vector<int32_t>& lvl, int W, const std::vector<int32_t>& cpw,
+                          int h, const int32_t* done, hipStream_t s) {
+  const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
+  if (cnt <= 0) return;
+  if (W == 64 && !cpw.empty() && cpw[h] > 0) {
+    const int c = cpw[h];
+    hipLaunchKernelGGL(k_outside_rows, dim3((cnt + 3) / 4, (F.max_chunks + c - 1) / c), dim3(256), 0, s, F, beg,
+                       cnt, c, done);
+    return;
+  }
+  LAML_DISPATCH_W(W, {
+    hipLaunchKernelGGL(k_outside<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt, done);
+  });
+}
+
+std::vector<int32_t> default_cpw(const std::vector<int32_t>& lvl, int W, int max_chunks) {
+  std::vector<int32_t> c;
+  if (W != 64) return c;
+  const int H = (int)lvl.size() - 1;
+  c.assign(H, 1);
+  for (int h = 1; h < H; ++h) {
+    const int64_t cnt = lvl[h + 1] - lvl[h];
+    // enough wavefronts to keep every SIMD busy (kRowTargetWaves), then as
+    // many chunks per wavefront as that allows
+    const int64_t want = cnt * max_chunks / kRowTargetWaves;
+    c[h] = (int32_t)std::max<int64_t>(1, std::min<int64_t>(max_chunks, want));
+  }
+  return c;
+}
+
+void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const std::vector<int32_t>& cpw,
+                const int32_t* done, hipStream_t s) {
   const int H = (int)lvl.size() - 1;
   const int ht = std::max(1, std::min(h_top, H));
-  for (int h = 1; h < ht; ++h) {
-    const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
-    if (cnt <= 0) continue;
-    LAML_DISPATCH_W(W, {
-      hipLaunchKernelGGL(k_inside<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt, done);
-    });
-  }
+  for (int h = 1; h < ht; ++h) inside_level(F, lvl, W, cpw, h, done, s);
   if (ht < H) {
     LAML_DISPATCH_W(W, {
       hipLaunchKernelGGL(k_inside_top<WW>, dim3(F.max_chunks), dim3(1024), 0, s, F, ht, H, done);
@@ -982,18 +1255,8 @@ void run_root(const DevForest& F, int W, double* site_llh_out, const int32_t* do
   });
 }
 
-static void inside_levels(const DevForest& F, const std::vector<int32_t>& lvl, int W, int a, int b,
-                          const int32_t* done, hipStream_t s) {
-  for (int h = a; h < b; ++h) {
-    const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
-    if (cnt <= 0) continue;
-    LAML_DISPATCH_W(W, {
-      hipLaunchKernelGGL(k_inside<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt, done);
-    });
-  }
-}
-
-void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W, int L, const int32_t* done,
+void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W, int L,
+                      const std::vector<int32_t>& in_cpw, const std::vector<int32_t>& out_cpw, const int32_t* done,
                       hipStream_t s) {
   const int H = (int)lvl.size() - 1;
   const int nseg = (H - 1 + L - 1) / L;  // internal levels 1..H-1
@@ -1001,19 +1264,14 @@ void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W
     const int a = 1 + sg * L, b = std::min(H, 1 + (sg + 1) * L);
     // recompute the segment's inside vectors (working area), then its
     // outside levels from the top of the segment down
-    if (sg != nseg - 1) inside_levels(F, lvl, W, a, b, done, s);
-    for (int h = b - 1; h >= a; --h) {
-      const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
-      if (cnt <= 0) continue;
-      LAML_DISPATCH_W(W, {
-        hipLaunchKernelGGL(k_outside<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt, done);
-      });
-    }
+    if (sg != nseg - 1)
+      for (int h = a; h < b; ++h) inside_level(F, lvl, W, in_cpw, h, done, s);
+    for (int h = b - 1; h >= a; --h) outside_level(F, lvl, W, out_cpw, h, done, s);
   }
 }
 
-void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const int32_t* done,
-                 hipStream_t s) {
+void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top,
+                 const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s) {
   const int H = (int)lvl.size() - 1;
   const int ht = std::max(1, std::min(h_top, H));
   if (ht < H) {
@@ -1021,13 +1279,7 @@ void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int
       hipLaunchKernelGGL(k_outside_top<WW>, dim3(F.max_chunks), dim3(1024), 0, s, F, ht, H, done);
     });
   }
-  for (int h = ht - 1; h >= 1; --h) {
-    const int beg = lvl[h], cnt = lvl[h + 1] - lvl[h];
-    if (cnt <= 0) continue;
-    LAML_DISPATCH_W(W, {
-      hipLaunchKernelGGL(k_outside<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt, done);
-    });
-  }
+  for (int h = ht - 1; h >= 1; --h) outside_level(F, lvl, W, cpw, h, done, s);
 }
 
 void run_counts(const DevForest& F, double* counts, hipStream_t s) {

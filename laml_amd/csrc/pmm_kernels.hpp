@@ -111,14 +111,21 @@ struct EmState {
 int seg_width(int max_k);
 void run_labels(const DevForest& F, const std::vector<int32_t>& lvl, int W, hipStream_t s);
 void run_coef(const DevForest& F, const double* d, const double* nu, const int32_t* done, hipStream_t s);
-void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const int32_t* done,
-                hipStream_t s);
-void run_root(const DevForest& F, int W, double* site_llh_out, const int32_t* done, hipStream_t s);
-void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const int32_t* done,
-                 hipStream_t s);
-void run_counts(const DevForest& F, double* counts, hipStream_t s);
-void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W, int L, const int32_t* done,
+// wavefronts a row-kernel launch aims for before it gives each wavefront
+// more than one chunk of 64 sites (8 per SIMD of the 256 CUs)
+constexpr int64_t kRowTargetWaves = 8192;
+// default chunks per wavefront of the row kernels for each level (empty
+// unless W == 64)
+std::vector<int32_t> default_cpw(const std::vector<int32_t>& lvl, int W, int max_chunks);
+void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const std::vector<int32_t>& cpw,
+                const int32_t* done, hipStream_t s);
+void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top,
+                 const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s);
+void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W, int L,
+                      const std::vector<int32_t>& in_cpw, const std::vector<int32_t>& out_cpw, const int32_t* done,
                       hipStream_t s);
+void run_root(const DevForest& F, int W, double* site_llh_out, const int32_t* done, hipStream_t s);
+void run_counts(const DevForest& F, double* counts, hipStream_t s);
 void run_llh(const DevForest& F, double* nll, EmState em, int step, int max_iter, double tol, hipStream_t s);
 void run_rtilde(const DevForest& F, double* rtilde, const int32_t* done, hipStream_t s);
 void run_grad(const DevForest& F, const double* d, const double* nu, const double* phi, double* grad_d,

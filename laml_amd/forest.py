@@ -403,6 +403,30 @@ class Forest(ForestLayout):
     def inside_rows(self):
         return int(self._lib.laml_forest_inside_rows(self.handle))
 
+    def get_shapes(self):
+        """Launch shapes: (inside_cpw, outside_cpw, top_level); the cpw arrays
+        hold, per level, the 64-site chunks one wavefront walks (all 0 when
+        the rows hold at most 32 sites)."""
+        H = self.num_levels
+        a = np.zeros(H, dtype=np.int32)
+        b = np.zeros(H, dtype=np.int32)
+        top = np.zeros(1, dtype=np.int32)
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        _lib.check(self._lib.laml_forest_get_shapes(self.handle, a.ctypes.data_as(i32), b.ctypes.data_as(i32),
+                                                    top.ctypes.data_as(i32)))
+        return a, b, int(top[0])
+
+    def set_shapes(self, inside_cpw=None, outside_cpw=None, top_level=None):
+        """Set launch shapes (None keeps the current value)."""
+        a0, b0, t0 = self.get_shapes()
+        a = np.ascontiguousarray(a0 if inside_cpw is None else inside_cpw, dtype=np.int32)
+        b = np.ascontiguousarray(b0 if outside_cpw is None else outside_cpw, dtype=np.int32)
+        if a.shape != (self.num_levels,) or b.shape != (self.num_levels,):
+            raise ValueError("shape arrays need one entry per level")
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        _lib.check(self._lib.laml_forest_set_shapes(self.handle, a.ctypes.data_as(i32), b.ctypes.data_as(i32),
+                                                    int(t0 if top_level is None else top_level)))
+
     def node_problem(self):
         """Problem index of every node (device int64 tensor, cached)."""
         if getattr(self, "_node_prob_t", None) is None:

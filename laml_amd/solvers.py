@@ -144,9 +144,7 @@ class EM_solver:
                 self._ultra_model = HeightModel(F, d)
                 self._ultra_key = key
                 F.set_ultrametric(self._ultra_model)
-            t0 = self._ultra_model.heights_from_lengths(d, self.dmin, self.dmax)
-            if not self._ultra_model.check_strict(t0, F.free, self.dmin, self.dmax):
-                return False
+            t0 = self._ultra_model.start_heights(d, F.free, self.dmin, self.dmax)
         except UltraInfeasible:
             return False
         F.set_heights(t0)

@@ -244,14 +244,14 @@ class HeightModel:
         d = self.lengths(t, len(free))[free.astype(bool)]
         return bool(np.all(d > dmin) and np.all(d < dmax))
 
-    def heights_from_lengths(self, d, dmin, dmax):
-        """A strictly feasible height vector: classes are placed bottom-up at
-        the largest height their children need (child branches at least
-        max(target length, 1.01 dmin)); raises if an upper bound fails."""
+    def heights_from_lengths(self, d, dmin, dmax, scale=1.0):
+        """Heights placed bottom-up: each class at the largest height its
+        child branches ask for, a child branch asking for
+        clip(scale * d, 1.01 dmin, 0.99 dmax).  Not necessarily strictly
+        feasible (an upper bound may fail); see ``start_heights``."""
         V = self.num_classes
         t = np.zeros(V)
         lo = np.full(V, -np.inf)
-        # lower bounds from children (branches whose parent side is the class)
         for c in range(V):
             for i in range(self.inc_off[c], self.inc_off[c + 1]):
                 v = self.inc_branch[i]
@@ -259,10 +259,22 @@ class HeightModel:
                     continue
                 dc = self.dn_cls[v]
                 tdn = (t[dc] + self.dn_off[v]) if dc >= 0 else self.dn_off[v]
-                want = max(float(d[v]), 1.01 * dmin)
-                want = min(want, 0.99 * dmax)
+                want = min(max(scale * float(d[v]), 1.01 * dmin), 0.99 * dmax)
                 lo[c] = max(lo[c], tdn + want - self.up_off[v])
             if not np.isfinite(lo[c]):
                 lo[c] = 0.0
             t[c] = lo[c]
         return t
+
+    def start_heights(self, d, free, dmin, dmax):
+        """A strictly feasible start for the barrier solver: the bottom-up
+        placement of ``heights_from_lengths`` with the wanted lengths scaled
+        down until every free branch lies strictly inside (dmin, dmax) (long
+        paths can push a branch to a pinned leaf beyond dmax).  The
+        constrained optimum does not depend on the start.  Raises
+        UltraInfeasible when even the shortest placement fails."""
+        for scale in (1.0, 0.5, 0.25, 0.1, 0.05, 0.02, 0.01, 0.0):
+            t = self.heights_from_lengths(d, dmin, dmax, scale)
+            if self.check_strict(t, free, dmin, dmax):
+                return t
+        raise UltraInfeasible("no strictly feasible ultrametric start")

@@ -51,6 +51,7 @@ DATASETS = {
     "example1": lambda: load_example1() + (0.1, 0.1),
     "syn250x30": lambda: _synthetic(250, 30, 5, 50, 1.0, 0.1, 0.1, 20261203) + (0.1, 0.1),
     "syn300x40_skew": lambda: _synthetic(300, 40, 2, 100, 0.5, 0.05, 0.2, 7) + (0.05, 0.2),
+    "syn200x150": lambda: _synthetic(200, 150, 5, 50, 1.0, 0.1, 0.1, 31) + (0.1, 0.1),
 }
 
 
@@ -304,3 +305,37 @@ def test_ultrametric_optimize_drop_in():
     print("ultra optimize", nllh_u, min(depths), max(depths), flush=True)
     assert max(depths) - min(depths) < 1e-8
     assert nllh_u >= 1007.009158767873 * (1 - 1e-6)
+
+
+def test_launch_shapes_agree():
+    """Every launch shape of the level passes (chunks per wavefront, fused
+    top levels on or off) gives the same likelihood and counts up to the
+    order of the sums over sites, and the oracle's likelihood."""
+    from laml_amd.forest import Forest
+
+    tree, Q, codes, _ = sim.make_problem(600, 200, 5, 50, 1.0, 0.1, 0.1, 77)
+    o = OracleEM([tree.newick()], sim.codes_to_charmtrx(tree, codes), Q, nu=0.1, phi=0.1)
+    want = o.negative_llh()
+    F = Forest([sim.problem_spec(tree, Q, codes)], device=torch.cuda.current_device())
+    d = torch.tensor(F.lengths_to_level([tree.length]), dtype=torch.float64, device=F.device)
+    nu = torch.full((1,), 0.1, dtype=torch.float64, device=F.device)
+    phi = torch.full((1,), 0.1, dtype=torch.float64, device=F.device)
+    a0, b0, top0 = F.get_shapes()
+    H = F.num_levels
+    assert a0[1:].min() >= 1  # 200 sites: row kernels
+    results = []
+    for cpw, top in [(None, None), (1, None), (4, None), (1, H), (4, 2)]:
+        if cpw is None:
+            F.set_shapes(a0, b0, top0)
+        else:
+            F.set_shapes(np.full(H, cpw), np.full(H, cpw), top if top is not None else top0)
+        counts, rt, nll = F.estep(d, nu, phi)
+        results.append((nll.item(), counts.cpu().numpy().copy(), rt.item()))
+        print("shape", cpw, top, nll.item(), want, flush=True)
+    for nll, counts, rt in results:
+        assert math.isclose(nll, want, rel_tol=NLL_RTOL)
+        assert math.isclose(nll, results[0][0], rel_tol=1e-13)
+        np.testing.assert_allclose(counts, results[0][1], rtol=1e-11, atol=1e-12)
+        assert math.isclose(rt, results[0][2], rel_tol=1e-12, abs_tol=1e-12)
+    with pytest.raises(Exception):
+        F.set_shapes(np.full(H, 99), None, None)

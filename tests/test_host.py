@@ -162,3 +162,38 @@ def test_yule_tree_is_ultrametric():
     leaves = [dist[v] for v in tr.traverse_leaves()]
     assert max(leaves) - min(leaves) < 1e-12
     assert abs(max(leaves) - 1.0) < 1e-12
+
+
+def test_ultrametric_start_heights_are_strictly_feasible():
+    """Random initial lengths (ML_solver.ini_brlens draws up to dmax/2 per
+    branch) can make the plain bottom-up placement exceed dmax on a branch to
+    a pinned leaf; the start construction must still be strictly feasible."""
+    import random
+    from helpers import load_topology_kats, rooted_topologies
+    from laml_amd.solvers import EM_solver
+    from laml_amd.ultra import HeightModel
+    Q, MSA, _, _ = load_topology_kats()
+    bad_plain = 0
+    for i, T in enumerate(rooted_topologies(list("abcde"))):
+        s = EM_solver([T], {'charMtrx': {**MSA["msa2"], 'e': [1, 0, 1, 0, 1]}}, {'Q': Q}, {'nu': 0, 'phi': 0})
+        random.seed(i)
+        s.x2params(s.ini_all())
+        L = ForestLayout([s._problem_spec(3)])
+        d = L.lengths_to_level([np.asarray([v.edge_length for v in t.traverse_postorder()]) for t in s.trees])
+        M = HeightModel(L, d)
+        t0 = M.heights_from_lengths(d, L.dmin, L.dmax)
+        bad_plain += not M.check_strict(t0, L.free, L.dmin, L.dmax)
+        t = M.start_heights(d, L.free, L.dmin, L.dmax)
+        assert M.check_strict(t, L.free, L.dmin, L.dmax)
+        lens = M.lengths(t, L.num_nodes)
+        # ultrametric: every leaf-to-root path has the same length
+        par = L.parent
+        depth = []
+        for leaf in range(L.num_leaves):
+            x, acc = leaf, 0.0
+            while x >= 0:
+                acc += lens[x]
+                x = par[x]
+            depth.append(acc)
+        assert max(depth) - min(depth) < 1e-9
+    assert bad_plain > 0  # the case the scaled placement exists for does occur
