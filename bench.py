@@ -101,15 +101,131 @@ def cpu_baseline(tree, Q, codes, nu, phi, target_s=12.0):
     }
 
 
+SCORING = {
+    # batched candidate scoring: NNI variants of one tree plus uneven
+    # independent trees, all with the same number of sites
+    "nni768": dict(base_n=2000, variants=512, others=256, n_lo=100, n_hi=5000, K=200, m_lo=5, m_hi=50,
+                   alpha=1.0, nu=0.1, phi=0.15, seed=20261208),
+}
+
+
+def bench_scoring(args):
+    """Batched scoring workload: one forest holding every candidate and
+    independent tree, EM from a uniform start, against the plain path of one
+    forest (one launch sequence) per tree."""
+    import torch
+
+    from laml_amd import sim
+    from laml_amd.autotune import autotune
+    from laml_amd.forest import Forest
+
+    c = SCORING[args.workload]
+    rng = np.random.Generator(np.random.PCG64(c["seed"]))
+    base_tree, Q, codes, _ = sim.make_problem(c["base_n"], c["K"], c["m_lo"], c["m_hi"], c["alpha"], c["nu"],
+                                              c["phi"], c["seed"])
+    problems = []
+    for i in range(c["variants"]):
+        t, cd = sim.nni_variant(base_tree, codes, rng)
+        problems.append((t, Q, cd))
+    sizes = np.exp(rng.uniform(np.log(c["n_lo"]), np.log(c["n_hi"]), c["others"])).astype(int)
+    for i, n in enumerate(sizes):
+        t, q, cd, _ = sim.make_problem(int(n), c["K"], c["m_lo"], c["m_hi"], c["alpha"], c["nu"], c["phi"],
+                                       c["seed"] + 1 + i)
+        problems.append((t, q, cd))
+    P = len(problems)
+    log("scoring workload %s: %d trees (%d NNI variants of a %d-leaf tree + %d trees of %d-%d leaves), K=%d" % (
+        args.workload, P, c["variants"], c["base_n"], c["others"], sizes.min(), sizes.max(), c["K"]))
+    torch.cuda.set_device(0)
+
+    def start(F, probs):
+        d = torch.tensor(F.lengths_to_level([np.full(t.n_nodes, 0.05) for (t, _, _) in probs]),
+                         dtype=torch.float64, device=F.device)
+        nu = torch.full((len(probs),), 0.5, dtype=torch.float64, device=F.device)
+        phi = torch.full((len(probs),), 0.5, dtype=torch.float64, device=F.device)
+        return d, nu, phi
+
+    specs = [sim.problem_spec(t, q, cd) for (t, q, cd) in problems]
+    t0 = time.perf_counter()
+    F = Forest(specs, device=0)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    d, nu, phi = start(F, problems)
+    plan = None
+    if not args.no_autotune:
+        plan, _ = autotune(F, d, nu, phi)
+    F.em(d, nu, phi, args.warmup, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nll, _ = F.em(d, nu, phi, args.steps, 0.0)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    batch_rate = P * args.steps / el
+    nll_b = nll.cpu().numpy().copy()
+    log("batched: %.3f ms per EM iteration of all %d trees, %.1f tree-iterations/s (setup %.1f s)" % (
+        el / args.steps * 1e3, P, batch_rate, setup))
+    F.close()
+    # plain path: one forest per tree, the same EM schedule, a sample of trees
+    sample = list(range(0, c["variants"], max(1, c["variants"] // 16))) + \
+        list(range(c["variants"], P, max(1, c["others"] // 16)))
+    per_tree = []
+    for i in sample:
+        Fi = Forest([specs[i]], device=0)
+        di, nui, phii = start(Fi, [problems[i]])
+        Fi.em(di, nui, phii, args.warmup, 0.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nli, _ = Fi.em(di, nui, phii, args.steps, 0.0)
+        torch.cuda.synchronize()
+        per_tree.append((time.perf_counter() - t0) / args.steps)
+        got = float(nli[0].item())
+        if abs(got - nll_b[i]) > 1e-9 * abs(got):
+            raise SystemExit("tree %d: batched result %r differs from alone %r" % (i, nll_b[i], got))
+        Fi.close()
+    # sampled mean time per tree-iteration, weighted to the workload's mix
+    nv = sum(1 for i in sample if i < c["variants"])
+    mean_v = float(np.mean(per_tree[:nv]))
+    mean_o = float(np.mean(per_tree[nv:]))
+    plain_s = c["variants"] * mean_v + c["others"] * mean_o
+    plain_rate = P / plain_s
+    log("one forest per tree: %.3f ms per variant iteration, %.3f ms per independent-tree iteration "
+        "(%d sampled); %.1f tree-iterations/s" % (mean_v * 1e3, mean_o * 1e3, len(sample), plain_rate))
+    res = {
+        "metric": "tree EM iterations/s, batched candidate scoring",
+        "value": round(batch_rate, 2),
+        "unit": "tree EM iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Yule trees, NNI variants, PMM-simulated characters)",
+        "config": {"workload": "%d NNI variants of a %d-leaf Yule tree + %d Yule trees with log-uniform %d-%d "
+                               "leaves, %d sites each, M in [%d,%d], nu=%g, phi=%g, one forest, EM from a "
+                               "uniform start" % (c["variants"], c["base_n"], c["others"], c["n_lo"], c["n_hi"],
+                                                  c["K"], c["m_lo"], c["m_hi"], c["nu"], c["phi"]),
+                   "trees": P, "nodes": int(sum(t.n_nodes for (t, _, _) in problems)),
+                   "launch_plan": plan.describe().get("mode") if plan else "default"},
+        "one_forest_per_tree": {"tree_iters_per_s": round(plain_rate, 2), "sampled_trees": len(sample),
+                                "speedup_of_batch": round(batch_rate / plain_rate, 2)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="em10k", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="em10k", choices=sorted(WORKLOADS) + sorted(SCORING))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-reps", type=int, default=10)
     ap.add_argument("--batch", type=int, default=16, help="independent problems per step")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch plan of the batch forest (no autotuning before timing)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the measurement, run 3 plain device copies of this many MiB (a known-size "
                          "reference for hardware byte counters)")
@@ -118,6 +234,9 @@ def main():
         print("bench.py measures one GPU: this build runs the PMM path on a single MI355X "
               "(see DESIGN.md for how it would shard); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.workload in SCORING:
+        return bench_scoring(args)
 
     import torch
 
@@ -131,7 +250,9 @@ def main():
     tree, Q, codes, truth = probs[0]
     torch.cuda.set_device(0)
 
-    def run(problems, steps, warmup):
+    plan_desc = {}
+
+    def run(problems, steps, warmup, tune=False):
         specs = [sim.problem_spec(t, q, c) for (t, q, c, _) in problems]
         t0 = time.perf_counter()
         F = Forest(specs, device=0)
@@ -142,6 +263,13 @@ def main():
         d = torch.tensor(d0, dtype=torch.float64, device=dev)
         nu = torch.full((len(problems),), 0.5, dtype=torch.float64, device=dev)
         phi = torch.full((len(problems),), 0.5, dtype=torch.float64, device=dev)
+        if tune:
+            from laml_amd.autotune import autotune
+            t1 = time.perf_counter()
+            plan, tlog = autotune(F, d, nu, phi)
+            plan_desc.update(plan.describe())
+            log("autotuned launch plan in %.2f s: %s (E-step %.3f ms -> %.3f ms)" % (
+                time.perf_counter() - t1, plan.describe().get("mode"), tlog[0][1], min(ms for _, ms in tlog)))
         if warmup > 0:
             F.em(d, nu, phi, warmup, 0.0)
         torch.cuda.synchronize()
@@ -159,7 +287,7 @@ def main():
     F1.close()
     del F1, d1, nu1, phi1
     # the batch
-    F, d, nu, phi, nll, el, setup_s = run(probs, args.steps, args.warmup)
+    F, d, nu, phi, nll, el, setup_s = run(probs, args.steps, args.warmup, tune=not args.no_autotune)
     em_per_s = B * args.steps / el
     final_nll = float(nll[0].item())
     log("batch of %d: %.3f ms/iter, %.1f aggregate EM it/s (x%.1f the single-problem rate); setup %.2f s" % (
@@ -212,7 +340,8 @@ def main():
         "config": {"workload": "fixed-topology PMM EM, batch of %d independent problems, each a %d-leaf Yule "
                                "tree x %d sites, M in [%d,%d], Dirichlet(%g), nu=%g, phi=%g, seeds %d+1000*b"
                                % (B, n, K, mlo, mhi, alpha, nu_true, phi_true, seed),
-                   "batch": B, "leaves": n, "sites": K, "nodes": F.num_nodes, "levels": F.num_levels},
+                   "batch": B, "leaves": n, "sites": K, "nodes": F.num_nodes, "levels": F.num_levels,
+                   "launch_plan": plan_desc or "default"},
         "node_site_evals_per_s": round(node_site * em_per_s, 1),
         "final_nllh_problem0": final_nll,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -119,6 +119,19 @@ int64_t laml_forest_inside_rows(laml_forest* f);
  * floating-point sums over sites only.
  * No counterpart in the reference (its passes are Python loops); the values
  * come from the default heuristic or from laml_amd.autotune. */
+/* Pass mode: LAML_PASS_LEVELS (default) runs each pass as level-synchronous
+ * launches over the whole forest; LAML_PASS_TREES runs the inside pass, the
+ * root step and the outside pass of every (tree, chunk of sites) pair in one
+ * block of a single launch, which suits forests of many small or uneven
+ * trees.  The tree walk needs full inside storage (no level checkpointing).
+ * Both modes compute the same values; only the order of floating-point sums
+ * over sites may differ.
+ * No counterpart in the reference. */
+#define LAML_PASS_LEVELS 0
+#define LAML_PASS_TREES 1
+int laml_forest_set_pass_mode(laml_forest* f, int32_t mode);
+int32_t laml_forest_pass_mode(laml_forest* f);
+
 int laml_forest_get_shapes(laml_forest* f, int32_t* inside_cpw, int32_t* outside_cpw, int32_t* top_level);
 int laml_forest_set_shapes(laml_forest* f, const int32_t* inside_cpw, const int32_t* outside_cpw,
                            int32_t top_level);

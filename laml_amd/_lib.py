@@ -12,6 +12,8 @@ LIB_PATH = os.path.join(_HERE, "liblaml_pmm.so")
 LAML_OK = 0
 LAML_OPT_PHI = 1
 LAML_OPT_NU = 2
+PASS_LEVELS = 0
+PASS_TREES = 1
 
 CODE_UNEDITED = 0
 CODE_MISSING = -1
@@ -92,6 +94,8 @@ SIGNATURES = {
                                ctypes.c_void_p, ctypes.c_void_p]),
     "laml_forest_set_ultrametric": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(UltraDesc)]),
     "laml_forest_set_heights": (ctypes.c_int, [ctypes.c_void_p, _f64p]),
+    "laml_forest_set_pass_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "laml_forest_pass_mode": (ctypes.c_int32, [ctypes.c_void_p]),
     "laml_forest_get_shapes": (ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p]),
     "laml_forest_set_shapes": (ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, ctypes.c_int32]),
     "laml_forest_ultra_status": (ctypes.c_int, [ctypes.c_void_p, _i32p]),

@@ -1,0 +1,102 @@
+"""Launch-plan autotuner for a device forest.
+
+A forest's passes can run as level-synchronous launches (per level: the
+number of 64-site chunks one wavefront walks, and whether the small top
+levels share one fused launch) or as the tree walk (one launch per pass, a
+block per (tree, site chunk)).  Which is fastest depends on the forest: how
+many trees, how deep, how many sites, how uneven.  ``autotune`` times the
+E-step (inside pass, root, outside pass, counts) under candidate plans on
+the forest itself and keeps the fastest; every plan computes the same values
+up to the order of floating-point sums over sites.
+
+The search is a coarse pass over pass mode and top-level fusion, then a
+coordinate search over the chunks per wavefront of the largest levels.
+"""
+import numpy as np
+import torch
+
+from ._lib import PASS_LEVELS, PASS_TREES
+
+
+def _time_estep(F, d, nu, phi, reps):
+    F.estep(d, nu, phi)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        F.estep(d, nu, phi)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+class Plan:
+    """A launch plan: pass mode, per-level chunks per wavefront, top level."""
+
+    def __init__(self, mode, inside_cpw, outside_cpw, top_level):
+        self.mode = int(mode)
+        self.inside_cpw = np.asarray(inside_cpw, dtype=np.int32).copy()
+        self.outside_cpw = np.asarray(outside_cpw, dtype=np.int32).copy()
+        self.top_level = int(top_level)
+
+    def apply(self, F):
+        F.set_pass_mode(PASS_LEVELS)
+        F.set_shapes(self.inside_cpw, self.outside_cpw, self.top_level)
+        F.set_pass_mode(self.mode)
+
+    def describe(self):
+        if self.mode == PASS_TREES:
+            return {"mode": "trees"}
+        return {"mode": "levels", "top_level": self.top_level,
+                "inside_cpw": self.inside_cpw.tolist(), "outside_cpw": self.outside_cpw.tolist()}
+
+
+def current_plan(F):
+    a, b, top = F.get_shapes()
+    return Plan(F.pass_mode(), a, b, top)
+
+
+def autotune(F, d, nu, phi, reps=3, max_levels=8, verbose=False):
+    """Time candidate plans on forest F with parameters (d, nu, phi) and apply
+    the fastest.  Returns (plan, log) where log lists (description, ms)."""
+    H = F.num_levels
+    base = current_plan(F)
+    log = []
+
+    def measure(plan, tag):
+        plan.apply(F)
+        ms = _time_estep(F, d, nu, phi, reps)
+        log.append((tag, ms))
+        if verbose:
+            print("autotune %-28s %.4f ms" % (tag, ms), flush=True)
+        return ms
+
+    best = base
+    best_ms = measure(base, "default")
+    cands = [("no fused top", Plan(PASS_LEVELS, base.inside_cpw, base.outside_cpw, H)),
+             ("tree walk", Plan(PASS_TREES, base.inside_cpw, base.outside_cpw, base.top_level))]
+    for tag, plan in cands:
+        ms = measure(plan, tag)
+        if ms < best_ms:
+            best, best_ms = plan, ms
+    rows = base.inside_cpw[1:].max() > 0  # row kernels in use (more than 32 sites)
+    if best.mode == PASS_LEVELS and rows:
+        sizes = np.diff(F.level_off)
+        big = [h for h in np.argsort(-sizes) if 1 <= h < min(H, best.top_level)][:max_levels]
+        max_chunks = int(max(best.inside_cpw.max(), best.outside_cpw.max(), 1))
+        options = sorted({1, 2, max(1, max_chunks // 2), max_chunks})
+        for which in ("outside_cpw", "inside_cpw"):
+            for h in big:
+                cur = int(getattr(best, which)[h])
+                for c in options:
+                    if c == cur:
+                        continue
+                    trial = Plan(best.mode, best.inside_cpw, best.outside_cpw, best.top_level)
+                    getattr(trial, which)[h] = c
+                    ms = measure(trial, "%s[%d]=%d" % (which, h, c))
+                    if ms < best_ms * 0.995:
+                        best, best_ms = trial, ms
+    best.apply(F)
+    return best, log

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -31,14 +32,40 @@ int fail(int code, const std::string& msg) {
 
 }  // namespace
 
+struct EmGraphKey {
+  const double *d, *nu, *phi, *nll, *trace;
+  double tol;
+  int32_t max_iter;
+  uint64_t plan;
+  bool operator==(const EmGraphKey& o) const {
+    return d == o.d && nu == o.nu && phi == o.phi && nll == o.nll && trace == o.trace && tol == o.tol &&
+           max_iter == o.max_iter && plan == o.plan;
+  }
+};
+
 struct laml_forest {
   int device = 0;
+  // captured EM step (replayed by laml_em) and the launch-plan version it
+  // was captured for; every change of launch shapes, checkpointing or the
+  // ultrametric model bumps the version
+  bool use_graph = true;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t em_exec = nullptr;
+  EmGraphKey em_key{};
+  uint64_t plan_version = 0;
+  void drop_graph() {
+    if (em_exec) (void)hipGraphExecDestroy(em_exec);
+    em_exec = nullptr;
+  }
   int W = 64;
   int h_top = 1 << 30;  // first level handled by the fused top-level kernels
   int h_top_default = 1 << 30;
   int ckpt_L = 0;       // level-checkpoint segment length (0: all inside vectors stored)
   // per level: 64-site chunks one wavefront of the row kernels walks (W == 64)
   std::vector<int32_t> in_cpw, out_cpw;
+  // pass mode: 0 = level-synchronous launches, 1 = tree walk (one launch)
+  int pass_mode = 0;
+  laml::TreeWalk TW{};
   // host copies for re-layout of the inside vectors
   std::vector<int32_t> h_parent;
   std::vector<int32_t> h_K;         // padded row stride of each node
@@ -67,6 +94,7 @@ struct laml_forest {
   std::vector<int32_t> ultra_status_h;
 
   void free_ultra() {
+    plan_version++;
     for (void* p : ultra_allocs) (void)hipFree(p);
     ultra_allocs.clear();
     ultra_on = false;
@@ -112,6 +140,8 @@ struct laml_forest {
     return LAML_OK;
   }
   ~laml_forest() {
+    drop_graph();
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (void* p : allocs) (void)hipFree(p);
     for (void* p : ultra_allocs) (void)hipFree(p);
     if (beta_alloc) (void)hipFree(beta_alloc);
@@ -329,6 +359,15 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   DeviceGuard guard(device);
   laml_forest* f = new laml_forest();
   f->device = device;
+  {
+    const char* env = getenv("LAML_EM_GRAPH");
+    f->use_graph = !(env && env[0] == '0');
+    // private capture stream (default priority)
+    if (hipStreamCreateWithFlags(&f->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+      delete f;
+      return fail(LAML_EDEVICE, "hipStreamCreate");
+    }
+  }
   f->W = laml::seg_width(max_k);
   f->level_off.assign(d->level_off, d->level_off + d->num_levels + 1);
   f->tree_nsites.assign(d->tree_nsites, d->tree_nsites + d->num_trees);
@@ -357,6 +396,40 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   F.max_chunks = (max_k + f->W - 1) / f->W;
   f->in_cpw = laml::default_cpw(f->level_off, f->W, F.max_chunks);
   f->out_cpw = f->in_cpw;
+  // tree-walk plan: each tree's per-level node ranges and its (tree, chunk)
+  // blocks, largest trees first
+  std::vector<int32_t> tw_blk_tree, tw_blk_chunk, tw_lvl_base(d->num_trees + 1, 0), tw_lvl_beg, tw_lvl_cnt;
+  {
+    const int T0 = d->num_trees, H = d->num_levels;
+    std::vector<int32_t> height(T0, 0), tsize(T0, 0);
+    for (int h = 0; h < H; ++h)
+      for (int v = d->level_off[h]; v < d->level_off[h + 1]; ++v) {
+        height[d->tree_of[v]] = h + 1;
+        tsize[d->tree_of[v]]++;
+      }
+    for (int t = 0; t < T0; ++t) tw_lvl_base[t + 1] = tw_lvl_base[t] + height[t];
+    tw_lvl_beg.assign(tw_lvl_base[T0], 0);
+    tw_lvl_cnt.assign(tw_lvl_base[T0], 0);
+    for (int h = 0; h < H; ++h)
+      for (int v = d->level_off[h]; v < d->level_off[h + 1]; ++v) {
+        const int t = d->tree_of[v];
+        const int i = tw_lvl_base[t] + h;
+        if (tw_lvl_cnt[i] == 0) tw_lvl_beg[i] = v;
+        tw_lvl_cnt[i]++;
+      }
+    std::vector<int32_t> order(T0);
+    for (int t = 0; t < T0; ++t) order[t] = t;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return (int64_t)tsize[a] * d->tree_nsites[a] > (int64_t)tsize[b] * d->tree_nsites[b];
+    });
+    for (int t : order) {
+      const int nch = (d->tree_nsites[t] + f->W - 1) / f->W;
+      for (int c = 0; c < nch; ++c) {
+        tw_blk_tree.push_back(t);
+        tw_blk_chunk.push_back(c);
+      }
+    }
+  }
   std::vector<int32_t> tree_prob(d->tree_problem, d->tree_problem + T);
 #define UP(dst, src, n) \
   if ((rc = f->upload(&F.dst, src, (size_t)(n)))) { delete f; return rc; }
@@ -386,6 +459,23 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   UP(prior, d->prior, d->site_qoff[S]);
   UP(codes, codes_p.data(), leaf_rows);
 #undef UP
+  {
+    const int32_t *bt = nullptr, *bc = nullptr, *lbase = nullptr, *lbeg = nullptr, *lcnt = nullptr;
+    if ((rc = f->upload(&bt, tw_blk_tree.data(), tw_blk_tree.size())) ||
+        (rc = f->upload(&bc, tw_blk_chunk.data(), tw_blk_chunk.size())) ||
+        (rc = f->upload(&lbase, tw_lvl_base.data(), tw_lvl_base.size())) ||
+        (rc = f->upload(&lbeg, tw_lvl_beg.data(), tw_lvl_beg.size())) ||
+        (rc = f->upload(&lcnt, tw_lvl_cnt.data(), tw_lvl_cnt.size()))) {
+      delete f;
+      return rc;
+    }
+    f->TW.num_blocks = (int32_t)tw_blk_tree.size();
+    f->TW.blk_tree = bt;
+    f->TW.blk_chunk = bc;
+    f->TW.lvl_base = lbase;
+    f->TW.lvl_beg = lbeg;
+    f->TW.lvl_cnt = lcnt;
+  }
 #define AL(dst, type, n) \
   if ((rc = f->alloc<type>(&dst, (size_t)(n)))) { delete f; return rc; }
   int16_t* labels;
@@ -432,6 +522,7 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   AL(f->phi_buf, double, P);
   AL(f->em.done, int32_t, P);
   AL(f->em.iters, int32_t, P);
+  AL(f->em.step, int32_t, P);
   AL(f->em.pre, double, P);
 #undef AL
   F.labels = labels;
@@ -524,6 +615,8 @@ int laml_forest_labels(laml_forest* f, int16_t* labels_host, int64_t capacity) {
 int laml_forest_set_checkpoint(laml_forest* f, int32_t levels) {
   if (!f) return fail(LAML_EINVAL, "null forest");
   if (levels < 0) return fail(LAML_EINVAL, "negative checkpoint length");
+  if (levels > 0 && f->pass_mode == LAML_PASS_TREES)
+    return fail(LAML_ESTATE, "level checkpointing needs the level pass mode");
   DeviceGuard guard(f->device);
   LAML_HIP(hipDeviceSynchronize());
   const int N = f->F.num_nodes, n = f->F.num_leaves;
@@ -589,6 +682,7 @@ int laml_forest_set_checkpoint(laml_forest* f, int32_t levels) {
   f->F.bE = static_cast<int32_t*>(ne);
   f->F.brow = db;
   f->ckpt_L = levels;
+  f->plan_version++;
   return LAML_OK;
 }
 
@@ -604,6 +698,18 @@ int laml_forest_get_shapes(laml_forest* f, int32_t* inside_cpw, int32_t* outside
   if (top_level) *top_level = f->h_top_default;
   return LAML_OK;
 }
+
+int laml_forest_set_pass_mode(laml_forest* f, int32_t mode) {
+  if (!f) return fail(LAML_EINVAL, "null forest");
+  if (mode != LAML_PASS_LEVELS && mode != LAML_PASS_TREES) return fail(LAML_EINVAL, "unknown pass mode");
+  if (mode == LAML_PASS_TREES && f->ckpt_L > 0)
+    return fail(LAML_ESTATE, "the tree walk needs full inside storage (level checkpointing is on)");
+  f->pass_mode = mode;
+  f->plan_version++;
+  return LAML_OK;
+}
+
+int32_t laml_forest_pass_mode(laml_forest* f) { return f ? f->pass_mode : -1; }
 
 int laml_forest_set_shapes(laml_forest* f, const int32_t* inside_cpw, const int32_t* outside_cpw, int32_t top_level) {
   if (!f) return fail(LAML_EINVAL, "null forest");
@@ -621,6 +727,7 @@ int laml_forest_set_shapes(laml_forest* f, const int32_t* inside_cpw, const int3
     }
   }
   f->h_top_default = top_level;
+  f->plan_version++;
   if (f->ckpt_L == 0) f->h_top = top_level;
   return LAML_OK;
 }
@@ -692,6 +799,7 @@ int laml_forest_set_ultrametric(laml_forest* f, const laml_ultra_desc* u) {
   LAML_HIP(hipMemset(U.t, 0, sizeof(double) * (V > 0 ? V : 1)));
   LAML_HIP(hipMemset(U.dt, 0, sizeof(double) * (V > 0 ? V : 1)));
   f->ultra_on = true;
+  f->plan_version++;
   return LAML_OK;
 }
 
@@ -718,6 +826,30 @@ static int check_launch() {
   return LAML_OK;
 }
 
+// inside pass and root step (site likelihoods) in the forest's pass mode
+static void pass_inside_root(laml_forest* f, const DevForest& F, double* site_llh, const int32_t* done,
+                             hipStream_t s) {
+  if (f->pass_mode == 1) {
+    laml::run_tree_walk(F, f->TW, f->W, 3, site_llh, done, s);
+  } else {
+    laml::run_inside(F, f->level_off, f->W, f->h_top, f->in_cpw, done, s);
+    laml::run_root(F, f->W, site_llh, done, s);
+  }
+}
+
+// outside pass with posterior partials (and the root step first when asked)
+static void pass_outside(laml_forest* f, const DevForest& F, bool with_root, const int32_t* done, hipStream_t s) {
+  if (f->pass_mode == 1) {
+    laml::run_tree_walk(F, f->TW, f->W, (with_root ? 2 : 0) | 4, nullptr, done, s);
+    return;
+  }
+  if (with_root) laml::run_root(F, f->W, nullptr, done, s);
+  if (f->ckpt_L > 0)
+    laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, f->in_cpw, f->out_cpw, done, s);
+  else
+    laml::run_outside(F, f->level_off, f->W, f->h_top, f->out_cpw, done, s);
+}
+
 int laml_nll(laml_forest* f, const double* d, const double* nu, const double* phi, double* nll, double* site_llh,
              void* stream) {
   if (!f || !d || !nu || !phi || !nll) return fail(LAML_EINVAL, "null argument");
@@ -726,10 +858,9 @@ int laml_nll(laml_forest* f, const double* d, const double* nu, const double* ph
   DevForest F = f->F;
   F.phi = phi;
   laml::run_coef(F, d, nu, nullptr, s);
-  laml::run_inside(F, f->level_off, f->W, f->h_top, f->in_cpw, nullptr, s);
-  laml::run_root(F, f->W, site_llh, nullptr, s);
+  pass_inside_root(f, F, site_llh, nullptr, s);
   EmState none{};
-  laml::run_llh(F, nll, none, 0, 0, 0.0, s);
+  laml::run_llh(F, nll, none, 0, 0.0, s);
   return check_launch();
 }
 
@@ -741,11 +872,14 @@ int laml_estep(laml_forest* f, const double* d, const double* nu, const double* 
   DevForest F = f->F;
   F.phi = phi;
   laml::run_coef(F, d, nu, nullptr, s);
-  laml::run_inside(F, f->level_off, f->W, f->h_top, f->in_cpw, nullptr, s);
-  laml::run_root(F, f->W, nullptr, nullptr, s);
+  if (f->pass_mode == 1) {
+    laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, nullptr, s);
+  } else {
+    pass_inside_root(f, F, nullptr, nullptr, s);
+    pass_outside(f, F, false, nullptr, s);
+  }
   EmState none{};
-  laml::run_llh(F, nll, none, 0, 0, 0.0, s);
-  (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, f->in_cpw, f->out_cpw, nullptr, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, f->out_cpw, nullptr, s));
+  laml::run_llh(F, nll, none, 0, 0.0, s);
   laml::run_counts(F, counts, s);
   laml::run_rtilde(F, rtilde, nullptr, s);
   return check_launch();
@@ -757,8 +891,7 @@ int laml_outside(laml_forest* f, const double* phi, double* counts, double* rtil
   hipStream_t s = static_cast<hipStream_t>(stream);
   DevForest F = f->F;
   F.phi = phi;
-  laml::run_root(F, f->W, nullptr, nullptr, s);
-  (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, f->in_cpw, f->out_cpw, nullptr, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, f->out_cpw, nullptr, s));
+  pass_outside(f, F, true, nullptr, s);
   laml::run_counts(F, counts, s);
   laml::run_rtilde(F, rtilde, nullptr, s);
   return check_launch();
@@ -771,8 +904,7 @@ int laml_grad(laml_forest* f, const double* d, const double* nu, const double* p
   hipStream_t s = static_cast<hipStream_t>(stream);
   DevForest F = f->F;
   F.phi = phi;
-  laml::run_root(F, f->W, nullptr, nullptr, s);
-  (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, f->in_cpw, f->out_cpw, nullptr, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, f->out_cpw, nullptr, s));
+  pass_outside(f, F, true, nullptr, s);
   laml::run_grad(F, d, nu, phi, grad_d, grad_nu, grad_phi, s);
   return check_launch();
 }
@@ -786,6 +918,26 @@ int laml_mstep(laml_forest* f, const double* counts, const double* rtilde, doubl
   return check_launch();
 }
 
+// one EM step of every problem not yet converged: inside pass, likelihood
+// and convergence test, then (unless this is the final step) outside pass
+// with posterior counts and the M-step
+static void em_step(laml_forest* f, const DevForest& F, const EmState& em, double* d, double* nu, double* phi,
+                    double* nll, int32_t max_iter, double tol, bool last, hipStream_t s) {
+  laml::run_coef(F, d, nu, em.done, s);
+  if (f->pass_mode == 1 && !last) {
+    // one launch for the whole E-step; the outside half of a problem that
+    // converges at this step is not used (its M-step is skipped)
+    laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, em.done, s);
+    laml::run_llh(F, nll, em, max_iter, tol, s);
+  } else {
+    pass_inside_root(f, F, nullptr, em.done, s);
+    laml::run_llh(F, nll, em, max_iter, tol, s);
+    if (last) return;
+    pass_outside(f, F, false, em.done, s);
+  }
+  laml::run_mstep(F, nullptr, nullptr, d, nu, phi, em.done, s, f->ultra_on ? &f->U : nullptr);
+}
+
 int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter, double tol, double* nll,
             int32_t* iters, double* trace, void* stream) {
   if (!f || !d || !nu || !phi || !nll || !iters) return fail(LAML_EINVAL, "null argument");
@@ -797,19 +949,43 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
   const int P = F.num_problems;
   EmState em = f->em;
   em.trace = trace;
-  LAML_HIP(hipMemsetAsync(em.done, 0, sizeof(int32_t) * P, s));
-  LAML_HIP(hipMemsetAsync(em.iters, 0, sizeof(int32_t) * P, s));
+  laml::run_em_reset(em, P, s);
+  // The EM step is captured once into a HIP graph and replayed: its kernel
+  // arguments do not change between steps (the step number lives on the
+  // device).  The graph holds kernel nodes only.
+  const bool graph = f->use_graph && max_iter > 0;
+  if (graph) {
+    // replayed steps all have step < max_iter, so the captured likelihood
+    // kernel gets a bound it never reaches and the graph serves any max_iter
+    const EmGraphKey key{d, nu, phi, nll, trace, tol, 0, f->plan_version};
+    if (!f->em_exec || !(f->em_key == key)) {
+      f->drop_graph();
+      hipGraph_t g = nullptr;
+      LAML_HIP(hipStreamBeginCapture(f->cap_stream, hipStreamCaptureModeThreadLocal));
+      em_step(f, F, em, d, nu, phi, nll, std::numeric_limits<int32_t>::max(), tol, false, f->cap_stream);
+      const hipError_t le = hipGetLastError();
+      const hipError_t ce = hipStreamEndCapture(f->cap_stream, &g);
+      if (le != hipSuccess || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(LAML_EDEVICE, std::string("EM step capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+      }
+      const hipError_t ie = hipGraphInstantiate(&f->em_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        f->em_exec = nullptr;
+        return fail(LAML_EDEVICE, std::string("EM graph instantiate: ") + hipGetErrorString(ie));
+      }
+      f->em_key = key;
+    }
+  }
   std::vector<int32_t> done_h(P);
   const int chunk = 8;
   for (int step = 0; step <= max_iter; ++step) {
-    laml::run_coef(F, d, nu, em.done, s);
-    laml::run_inside(F, f->level_off, f->W, f->h_top, f->in_cpw, em.done, s);
-    laml::run_root(F, f->W, nullptr, em.done, s);
-    laml::run_llh(F, nll, em, step, max_iter, tol, s);
-    if (step < max_iter) {
-      (f->ckpt_L > 0 ? laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, f->in_cpw, f->out_cpw, em.done, s) : laml::run_outside(F, f->level_off, f->W, f->h_top, f->out_cpw, em.done, s));
-      laml::run_mstep(F, nullptr, nullptr, d, nu, phi, em.done, s, f->ultra_on ? &f->U : nullptr);
-    }
+    const bool last = step == max_iter;
+    if (graph && !last)
+      LAML_HIP(hipGraphLaunch(f->em_exec, s));
+    else
+      em_step(f, F, em, d, nu, phi, nll, max_iter, tol, last, s);
     int rc = check_launch();
     if (rc) return rc;
     if (tol > 0.0 && step % chunk == chunk - 1 && step < max_iter) {

@@ -264,19 +264,16 @@ __global__ void __launch_bounds__(1024) k_inside_top(DevForest F, int h0, int h1
   }
 }
 
-// Root of every tree: site likelihood L = P(D | stem starts unedited), the
-// outside state at the root, and the root branch's posterior counts (one
-// partial per site chunk).  Segments index trees.
+// Root of tree t at the sites of one chunk: site likelihood L = P(D | stem
+// starts unedited), the outside state at the root, and the root branch's
+// posterior counts (partial number `chunk`).  All W lanes of the segment call
+// this together.
 template <int W>
-__global__ void __launch_bounds__(256) k_root(DevForest F, double* __restrict__ site_llh_out,
-                                              const int32_t* __restrict__ done) {
-  Seg<W> g;
-  if (!g.init(0, F.num_trees)) return;
-  const int t = g.v, k = g.k, lane = g.lane;
+__device__ __forceinline__ void root_chunk(const DevForest& F, int t, int chunk, int lane,
+                                           double* __restrict__ site_llh_out) {
   const int p = F.tree_prob[t];
   const int K = F.tree_nsites[t];
-  if ((int)blockIdx.y * W >= K) return;
-  if (done && done[p]) return;
+  const int k = chunk * W + lane;
   const int r = F.tree_root[t];
   const bool act = k < K;
   double acc0 = 0, acc1 = 0, acc2 = 0;
@@ -320,13 +317,25 @@ __global__ void __launch_bounds__(256) k_root(DevForest F, double* __restrict__ 
   acc1 = seg_sum_last<W>(acc1);
   acc2 = seg_sum_last<W>(acc2);
   if (lane == W - 1) {
-    double* c = F.cpart + ((int64_t)r * F.max_chunks + blockIdx.y) * 5;
+    double* c = F.cpart + ((int64_t)r * F.max_chunks + chunk) * 5;
     c[0] = acc0;
     c[1] = acc1;
     c[2] = acc2;
     c[3] = 0.0;
     c[4] = 0.0;
   }
+}
+
+// Root step of every tree (segments index trees, blockIdx.y the site chunk).
+template <int W>
+__global__ void __launch_bounds__(256) k_root(DevForest F, double* __restrict__ site_llh_out,
+                                              const int32_t* __restrict__ done) {
+  Seg<W> g;
+  if (!g.init(0, F.num_trees)) return;
+  const int t = g.v;
+  if ((int)blockIdx.y * W >= F.tree_nsites[t]) return;
+  if (done && done[F.tree_prob[t]]) return;
+  root_chunk<W>(F, t, (int)blockIdx.y, g.lane, site_llh_out);
 }
 
 // Outside pass for the children of the internal nodes of one level, with the
@@ -573,6 +582,70 @@ __global__ void __launch_bounds__(1024) k_outside_top(DevForest F, int h0, int h
 
 
 
+// threads per tree-walk block (16 wavefronts share each level of the tree)
+constexpr int kWalkThreads = 1024;
+
+// Tree walk: one block per (tree, chunk of W sites) runs the whole pass of
+// that tree at those sites -- inside levels bottom-up, the root step, outside
+// levels top-down -- with block barriers between levels.  Sites of one chunk
+// never depend on other chunks, so blocks are independent: a forest of many
+// trees of uneven sizes is one launch per pass, and each block stops at its
+// own tree's height.  phases: 1 inside, 2 root, 4 outside (any combination,
+// in that order).
+template <int W>
+__global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWalk TW, int phases,
+                                                            double* __restrict__ site_llh_out,
+                                                            const int32_t* __restrict__ done) {
+  const int t = TW.blk_tree[blockIdx.x];
+  const int chunk = TW.blk_chunk[blockIdx.x];
+  const int p = F.tree_prob[t];
+  if (done && done[p]) return;
+  const int K = F.tree_nsites[t];
+  constexpr int kSegs = kWalkThreads / W;
+  const int seg = (int)(threadIdx.x / W);
+  const int lane = threadIdx.x & (W - 1);
+  const int k = chunk * W + lane;
+  const int lb = TW.lvl_base[t];
+  const int H = TW.lvl_base[t + 1] - lb;
+  if (phases & 1) {
+    for (int h = 1; h < H; ++h) {
+      const int beg = TW.lvl_beg[lb + h], cnt = TW.lvl_cnt[lb + h];
+      for (int i = seg; i < cnt; i += kSegs) {
+        const int v = W == 64 ? __builtin_amdgcn_readfirstlane(beg + i) : beg + i;
+        if (k < K) {
+          InNode n;
+          in_node_load(F, v, t, p, K, n);
+          inside_site<-1, -1>(F, n, k);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (phases & 2) {
+    if (seg == 0) root_chunk<W>(F, t, chunk, lane, site_llh_out);
+    __syncthreads();
+  }
+  if (phases & 4) {
+    for (int h = H - 1; h >= 1; --h) {
+      const int beg = TW.lvl_beg[lb + h], cnt = TW.lvl_cnt[lb + h];
+      for (int i = seg; i < cnt; i += kSegs) {
+        const int u = W == 64 ? __builtin_amdgcn_readfirstlane(beg + i) : beg + i;
+        OutNode n;
+        out_node_load(F, u, t, p, K, n);
+        double acc[2][5];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int j = 0; j < 5; ++j) acc[a][j] = 0.0;
+        double rt = 0.0;
+        if (k < K) outside_site<-1, -1>(F, n, k, acc, rt);
+        out_write_partials<W>(F, u, n.ch, lane, chunk, chunk + 1, acc, rt);
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // Per-node counts: fixed-order sum of the per-chunk partials.
 __global__ void k_counts(DevForest F, double* __restrict__ counts) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -604,8 +677,12 @@ __device__ __forceinline__ double block_sum(double x, double* sh) {
 
 // Sum of per-site log-likelihoods of every tree of a problem, plus the EM
 // bookkeeping of the device loop when em.done != nullptr.
-__global__ void __launch_bounds__(256) k_llh(DevForest F, double* __restrict__ nll, EmState em, int step,
-                                             int max_iter, double tol) {
+// Per-problem log-likelihood; inside the EM loop also the convergence test
+// of EM_solver.EM_optimization (EM_solver.py:483-512).  The EM step number of
+// each problem lives on the device (em.step), so that a captured EM step can
+// be replayed unchanged.
+__global__ void __launch_bounds__(256) k_llh(DevForest F, double* __restrict__ nll, EmState em, int max_iter,
+                                             double tol) {
   __shared__ double sh[4];
   const int p = blockIdx.x;
   if (em.done && em.done[p]) return;
@@ -619,6 +696,8 @@ __global__ void __launch_bounds__(256) k_llh(DevForest F, double* __restrict__ n
   if (threadIdx.x != 0) return;
   if (nll) nll[p] = -llh;
   if (!em.done) return;
+  const int step = em.step[p];
+  em.step[p] = step + 1;
   if (em.trace) em.trace[(int64_t)step * F.num_problems + p] = -llh;
   if (step > 0) {
     const double pre = em.pre[p];
@@ -1282,12 +1361,34 @@ void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int
   for (int h = ht - 1; h >= 1; --h) outside_level(F, lvl, W, cpw, h, done, s);
 }
 
+void run_tree_walk(const DevForest& F, const TreeWalk& TW, int W, int phases, double* site_llh_out,
+                   const int32_t* done, hipStream_t s) {
+  LAML_DISPATCH_W(W, {
+    hipLaunchKernelGGL(k_tree_walk<WW>, dim3(TW.num_blocks), dim3(kWalkThreads), 0, s, F, TW, phases, site_llh_out,
+                       done);
+  });
+}
+
 void run_counts(const DevForest& F, double* counts, hipStream_t s) {
   hipLaunchKernelGGL(k_counts, dim3((F.num_nodes + 255) / 256), dim3(256), 0, s, F, counts);
 }
 
-void run_llh(const DevForest& F, double* nll, EmState em, int step, int max_iter, double tol, hipStream_t s) {
-  hipLaunchKernelGGL(k_llh, dim3(F.num_problems), dim3(256), 0, s, F, nll, em, step, max_iter, tol);
+// clears the EM bookkeeping of every problem (a kernel rather than a memset,
+// so that the call stays valid inside a caller's stream capture)
+__global__ void k_em_reset(EmState em, int P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  em.done[p] = 0;
+  em.iters[p] = 0;
+  em.step[p] = 0;
+}
+
+void run_em_reset(const EmState& em, int P, hipStream_t s) {
+  hipLaunchKernelGGL(k_em_reset, dim3((P + 255) / 256), dim3(256), 0, s, em, P);
+}
+
+void run_llh(const DevForest& F, double* nll, EmState em, int max_iter, double tol, hipStream_t s) {
+  hipLaunchKernelGGL(k_llh, dim3(F.num_problems), dim3(256), 0, s, F, nll, em, max_iter, tol);
 }
 
 void run_grad(const DevForest& F, const double* d, const double* nu, const double* phi, double* grad_d,

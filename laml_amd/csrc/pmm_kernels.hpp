@@ -101,9 +101,23 @@ struct UltraDev {
   int32_t* status;           // [P] 0 ok, 1 line search stalled, 2 singular Newton system
 };
 
+// Tree-walk launch plan: block b runs tree blk_tree[b] at site chunk
+// blk_chunk[b]; tree t's internal levels are lvl_base[t] + 1 ..
+// lvl_base[t+1] - 1, level h holding the contiguous node range
+// [lvl_beg[lvl_base[t] + h], + lvl_cnt[...]).
+struct TreeWalk {
+  int32_t num_blocks;
+  const int32_t* blk_tree;
+  const int32_t* blk_chunk;
+  const int32_t* lvl_base;  // [T+1]
+  const int32_t* lvl_beg;
+  const int32_t* lvl_cnt;
+};
+
 struct EmState {
   int32_t* done;   // [P] or nullptr outside the EM loop
   int32_t* iters;  // [P]
+  int32_t* step;   // [P] EM steps taken (read and advanced on the device)
   double* pre;     // [P]
   double* trace;   // [(max_iter+1)*P] or nullptr
 };
@@ -126,7 +140,10 @@ void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W
                       hipStream_t s);
 void run_root(const DevForest& F, int W, double* site_llh_out, const int32_t* done, hipStream_t s);
 void run_counts(const DevForest& F, double* counts, hipStream_t s);
-void run_llh(const DevForest& F, double* nll, EmState em, int step, int max_iter, double tol, hipStream_t s);
+void run_tree_walk(const DevForest& F, const TreeWalk& TW, int W, int phases, double* site_llh_out,
+                   const int32_t* done, hipStream_t s);
+void run_llh(const DevForest& F, double* nll, EmState em, int max_iter, double tol, hipStream_t s);
+void run_em_reset(const EmState& em, int P, hipStream_t s);
 void run_rtilde(const DevForest& F, double* rtilde, const int32_t* done, hipStream_t s);
 void run_grad(const DevForest& F, const double* d, const double* nu, const double* phi, double* grad_d,
               double* grad_nu, double* grad_phi, hipStream_t s);

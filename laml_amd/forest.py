@@ -403,6 +403,14 @@ class Forest(ForestLayout):
     def inside_rows(self):
         return int(self._lib.laml_forest_inside_rows(self.handle))
 
+    def set_pass_mode(self, mode):
+        """0 (levels) or 1 (tree walk: one launch per pass, a block per
+        (tree, site chunk))."""
+        _lib.check(self._lib.laml_forest_set_pass_mode(self.handle, int(mode)))
+
+    def pass_mode(self):
+        return int(self._lib.laml_forest_pass_mode(self.handle))
+
     def get_shapes(self):
         """Launch shapes: (inside_cpw, outside_cpw, top_level); the cpw arrays
         hold, per level, the 64-site chunks one wavefront walks (all 0 when

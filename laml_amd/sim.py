@@ -208,3 +208,41 @@ def problem_spec(tree, Q, codes, lengths=None, free=None, flags=3):
     nobs = float((codes != -1).sum())
     phimin = 1.0 / (len(Q) * codes.shape[0])
     return ProblemSpec([ta], sites, nobs, phimin, flags)
+
+
+def nni_variant(tree, codes, rng):
+    """The tree after one random NNI move (a child of a random internal
+    non-root node u swapped with u's sibling), as a new YuleTree in post-order
+    with the code rows permuted to its leaf order."""
+    T = tree.to_tree()
+    internal = [v for v in T.traverse_postorder() if v.children and v.parent is not None]
+    if not internal:
+        raise ValueError("tree has no internal non-root branch")
+    u = internal[int(rng.integers(len(internal)))]
+    v = u.parent
+    w = v.children[0] if v.children[1] is u else v.children[1]
+    c = u.children[int(rng.integers(2))]
+    u.children[u.children.index(c)] = w
+    w.parent = u
+    v.children[v.children.index(w)] = c
+    c.parent = v
+    nodes = list(T.traverse_postorder())
+    idx = {id(x): i for i, x in enumerate(nodes)}
+    N = len(nodes)
+    c0 = np.full(N, -1, dtype=np.int64)
+    c1 = np.full(N, -1, dtype=np.int64)
+    parent = np.full(N, -1, dtype=np.int64)
+    length = np.zeros(N)
+    names = []
+    for i, x in enumerate(nodes):
+        length[i] = x.edge_length
+        if x.children:
+            c0[i] = idx[id(x.children[0])]
+            c1[i] = idx[id(x.children[1])]
+        else:
+            names.append(x.label)
+        if x.parent is not None:
+            parent[i] = idx[id(x.parent)]
+    row = {name: i for i, name in enumerate(tree.leaf_names)}
+    perm = np.asarray([row[nm] for nm in names], dtype=np.int64)
+    return YuleTree(c0, c1, parent, length, names), codes[perm]

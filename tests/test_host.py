@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_declared_symbol():
     lib = _lib.load()
     header = open(os.path.join(ROOT, "include", "laml_pmm.h")).read()
-    names = set(re.findall(r"^(?:int|int64_t|void|const char\*)\s+(laml_\w+)\(", header, re.M))
+    names = set(re.findall(r"^(?:int|int32_t|int64_t|void|const char\*)\s+(laml_\w+)\(", header, re.M))
     assert names, "no declarations parsed"
     assert names == set(_lib.SIGNATURES), names ^ set(_lib.SIGNATURES)
     for n in names:
