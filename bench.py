@@ -109,6 +109,107 @@ SCORING = {
 }
 
 
+SERVING = {
+    # Poisson stream of likelihood + EM requests of mixed sizes
+    "serve": dict(requests=48, n_lo=200, n_hi=50000, K_lo=20, K_hi=500, nu=(0.02, 0.2), phi=(0.05, 0.3),
+                  em_iters=50, tol=1e-8, slots=4, loads=(0.5, 1.0, 2.0), seed=20261209),
+}
+
+
+def bench_serving(args):
+    """Serving workload: requests arrive as a Poisson stream at a rate set
+    relative to the plain path's capacity; the continuous-batching server
+    (concurrent slots, admission by device footprint) against the plain path
+    (one request at a time, in arrival order)."""
+    import threading
+
+    import torch
+
+    from laml_amd import sim
+    from laml_amd.serving import PMMServer, SolveRequest, latency_summary, solve_alone
+
+    c = SERVING[args.workload]
+    rng = np.random.Generator(np.random.PCG64(c["seed"]))
+    reqs = []
+    for i in range(c["requests"]):
+        n = int(np.exp(rng.uniform(np.log(c["n_lo"]), np.log(c["n_hi"]))))
+        K = int(rng.integers(c["K_lo"], c["K_hi"] + 1))
+        nu = float(rng.uniform(*c["nu"]))
+        phi = float(rng.uniform(*c["phi"]))
+        tree, Q, codes, _ = sim.make_problem(n, K, 5, 50, 1.0, nu, phi, c["seed"] + 1 + i)
+        reqs.append((sim.problem_spec(tree, Q, codes), tree.n_nodes))
+    log("serving workload: %d requests, leaves %d-%d, sites %d-%d, %d EM iterations each" % (
+        len(reqs), c["n_lo"], c["n_hi"], c["K_lo"], c["K_hi"], c["em_iters"]))
+    torch.cuda.set_device(0)
+
+    def make(i):
+        spec, nn = reqs[i]
+        return SolveRequest(spec, [np.full(nn, 0.05)], 0.5, 0.5, max_iter=c["em_iters"], tol=c["tol"], rid=i)
+
+    # warm the code paths, then the plain path's service times
+    solve_alone(make(0))
+    torch.cuda.synchronize()
+    service = []
+    for i in range(len(reqs)):
+        t0 = time.perf_counter()
+        solve_alone(make(i))
+        service.append(time.perf_counter() - t0)
+    capacity = len(reqs) / sum(service)
+    log("plain path: mean service %.1f ms, capacity %.2f req/s" % (1e3 * np.mean(service), capacity))
+
+    def replay(slots, rate, seed):
+        arr = np.cumsum(np.random.Generator(np.random.PCG64(seed)).exponential(1.0 / rate, len(reqs)))
+        results = [None] * len(reqs)
+        with PMMServer(slots=slots) as srv:
+            t0 = time.perf_counter()
+            futs = []
+            for i, ta in enumerate(arr):
+                dt = t0 + ta - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                futs.append(srv.submit(make(i)))
+            for i, f in enumerate(futs):
+                results[i] = f.result()
+        return latency_summary(results), srv.peak_active
+
+    runs = []
+    for load in c["loads"]:
+        rate = load * capacity
+        served, peak = replay(c["slots"], rate, c["seed"] + 7)
+        plain, _ = replay(1, rate, c["seed"] + 7)
+        log("offered load %.1fx plain capacity (%.2f req/s): served p50 %.0f ms p99 %.0f ms %.2f req/s "
+            "(peak %d concurrent); plain p50 %.0f ms p99 %.0f ms %.2f req/s" % (
+                load, rate, served["p50_ms"], served["p99_ms"], served["throughput_req_per_s"], peak,
+                plain["p50_ms"], plain["p99_ms"], plain["throughput_req_per_s"]))
+        runs.append({"offered_load": load, "rate_req_per_s": round(rate, 3),
+                     "served": {k: round(v, 3) for k, v in served.items()},
+                     "plain": {k: round(v, 3) for k, v in plain.items()}, "peak_concurrent": peak})
+    top = runs[-1]
+    res = {
+        "metric": "p99 solve latency at %.1fx the plain path's capacity" % top["offered_load"],
+        "value": top["served"]["p99_ms"],
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": c["requests"],
+        "warmup": 1,
+        "ms_per_step": round(1e3 / top["served"]["throughput_req_per_s"], 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Yule trees, PMM-simulated characters, Poisson arrivals)",
+        "config": {"workload": "%d requests, leaves log-uniform %d-%d, sites uniform %d-%d, nu in [%g,%g], "
+                               "phi in [%g,%g], %d EM iterations (tol %g) from a uniform start; %d slots, "
+                               "admission by device footprint" % (
+                                   c["requests"], c["n_lo"], c["n_hi"], c["K_lo"], c["K_hi"], c["nu"][0],
+                                   c["nu"][1], c["phi"][0], c["phi"][1], c["em_iters"], c["tol"], c["slots"]),
+                   "plain_capacity_req_per_s": round(capacity, 3)},
+        "runs": runs,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def bench_scoring(args):
     """Batched scoring workload: one forest holding every candidate and
     independent tree, EM from a uniform start, against the plain path of one
@@ -220,7 +321,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="em10k", choices=sorted(WORKLOADS) + sorted(SCORING))
+    ap.add_argument("--workload", default="em10k", choices=sorted(WORKLOADS) + sorted(SCORING) + sorted(SERVING))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-reps", type=int, default=10)
     ap.add_argument("--batch", type=int, default=16, help="independent problems per step")
@@ -237,6 +338,8 @@ def main():
 
     if args.workload in SCORING:
         return bench_scoring(args)
+    if args.workload in SERVING:
+        return bench_serving(args)
 
     import torch
 

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <limits>
+#include <mutex>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -18,6 +19,11 @@ using laml::EmState;
 namespace {
 
 thread_local std::string g_err;
+// Serialises the HIP graph capture of laml_em with every synchronous
+// operation on the legacy default stream (the uploads and read-backs of the
+// set-up calls): such an operation must not run while another thread
+// captures, or it would have to depend on the capturing stream.
+std::mutex g_legacy_mu;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -319,6 +325,7 @@ int laml_forest_validate(const laml_forest_desc* d) {
 }
 
 int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out) {
+  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
   if (!out) return fail(LAML_EINVAL, "null output pointer");
   *out = nullptr;
   Derived D;
@@ -574,6 +581,7 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
 }
 
 void laml_forest_destroy(laml_forest* f) {
+  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
   if (!f) return;
   DeviceGuard guard(f->device);
   (void)hipDeviceSynchronize();
@@ -581,6 +589,7 @@ void laml_forest_destroy(laml_forest* f) {
 }
 
 int laml_forest_set_flags(laml_forest* f, const uint32_t* flags_host) {
+  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
   if (!f || !flags_host) return fail(LAML_EINVAL, "null argument");
   DeviceGuard guard(f->device);
   LAML_HIP(hipMemcpy(const_cast<uint32_t*>(f->F.prob_flags), flags_host, sizeof(uint32_t) * f->F.num_problems,
@@ -589,6 +598,7 @@ int laml_forest_set_flags(laml_forest* f, const uint32_t* flags_host) {
 }
 
 int laml_forest_labels(laml_forest* f, int16_t* labels_host, int64_t capacity) {
+  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
   if (!f || !labels_host) return fail(LAML_EINVAL, "null argument");
   DeviceGuard guard(f->device);
   const int64_t need = f->leaf_sites + f->int_sites;
@@ -613,6 +623,7 @@ int laml_forest_labels(laml_forest* f, int16_t* labels_host, int64_t capacity) {
 }
 
 int laml_forest_set_checkpoint(laml_forest* f, int32_t levels) {
+  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
   if (!f) return fail(LAML_EINVAL, "null forest");
   if (levels < 0) return fail(LAML_EINVAL, "negative checkpoint length");
   if (levels > 0 && f->pass_mode == LAML_PASS_TREES)
@@ -733,6 +744,7 @@ int laml_forest_set_shapes(laml_forest* f, const int32_t* inside_cpw, const int3
 }
 
 int laml_forest_set_ultrametric(laml_forest* f, const laml_ultra_desc* u) {
+  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
   if (!f) return fail(LAML_EINVAL, "null forest");
   DeviceGuard guard(f->device);
   LAML_HIP(hipDeviceSynchronize());
@@ -804,6 +816,7 @@ int laml_forest_set_ultrametric(laml_forest* f, const laml_ultra_desc* u) {
 }
 
 int laml_forest_set_heights(laml_forest* f, const double* heights_host) {
+  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
   if (!f || !heights_host) return fail(LAML_EINVAL, "null argument");
   if (!f->ultra_on) return fail(LAML_ESTATE, "no ultrametric model set");
   DeviceGuard guard(f->device);
@@ -813,6 +826,7 @@ int laml_forest_set_heights(laml_forest* f, const double* heights_host) {
 }
 
 int laml_forest_ultra_status(laml_forest* f, int32_t* status_host) {
+  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
   if (!f || !status_host) return fail(LAML_EINVAL, "null argument");
   if (!f->ultra_on) return fail(LAML_ESTATE, "no ultrametric model set");
   DeviceGuard guard(f->device);
@@ -959,6 +973,7 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
     // kernel gets a bound it never reaches and the graph serves any max_iter
     const EmGraphKey key{d, nu, phi, nll, trace, tol, 0, f->plan_version};
     if (!f->em_exec || !(f->em_key == key)) {
+      std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
       f->drop_graph();
       hipGraph_t g = nullptr;
       LAML_HIP(hipStreamBeginCapture(f->cap_stream, hipStreamCaptureModeThreadLocal));

@@ -1,0 +1,48 @@
+"""Serving loop on the GPU: every request's result is the same, bit for bit,
+whether it is solved alone or in a mix of concurrent requests of different
+sizes admitted under a memory budget that forces queueing."""
+import numpy as np
+import pytest
+import torch
+
+from laml_amd import sim
+from laml_amd.serving import PMMServer, SolveRequest, latency_summary, solve_alone
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    out = []
+    for i, (n, K) in enumerate([(200, 20), (3000, 150), (800, 40), (50, 8), (5000, 300), (1200, 64),
+                                (400, 33), (2500, 100), (150, 480), (900, 12)]):
+        nu, phi = 0.02 + 0.02 * i, 0.05 + 0.025 * i
+        tree, Q, codes, _ = sim.make_problem(n, K, 5, 40, 1.0, nu, phi, 4000 + i)
+        spec = sim.problem_spec(tree, Q, codes)
+        out.append(SolveRequest(spec, [np.full(tree.n_nodes, 0.05)], 0.5, 0.5, max_iter=60, tol=1e-8, rid=i))
+    return out
+
+
+def test_served_results_equal_solving_alone():
+    reqs = _requests()
+    alone = {r.rid: solve_alone(r) for r in reqs}
+    sizes = sorted(r.bytes for r in reqs)
+    budget = sum(sizes[-3:])  # the largest requests cannot all be resident together
+    with PMMServer(slots=4, hbm_budget=budget) as srv:
+        futs = [srv.submit(r) for r in reqs]
+        res = [f.result(timeout=300) for f in futs]
+    assert srv.peak_in_use <= budget and srv.peak_active >= 2
+    for r in res:
+        a = alone[r.rid]
+        print("request", r.rid, r.nll, a[0], r.iters, flush=True)
+        assert r.nll == a[0] and r.nu == a[2] and r.phi == a[3] and r.iters == a[4]
+        for x, y in zip(r.lengths, a[1]):
+            np.testing.assert_array_equal(x, y)
+    s = latency_summary(res)
+    assert s["p99_ms"] >= s["p50_ms"] > 0
+
+
+def test_request_over_budget_rejected():
+    r = _requests()[4]
+    with PMMServer(slots=1, hbm_budget=r.bytes - 1) as srv:
+        with pytest.raises(ValueError):
+            srv.submit(r)
