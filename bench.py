@@ -59,6 +59,21 @@ def algorithmic_bytes(F):
     return inside, outside
 
 
+PROFILE = "profiles/outside_pass_em10k_batch16.md"
+
+
+def outside_traffic_from_profile(args):
+    """HBM bytes of one outside pass of this workload, as measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy's
+    counts) and recorded in the committed profile summary; None if absent."""
+    import re
+    path = os.path.join(ROOT, PROFILE)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"measured traffic total \| ([\d.]+) ", open(path).read())
+    return int(float(m.group(1)) * 1e9) if m else None
+
+
 def cpu_baseline(tree, Q, codes, nu, phi, target_s=12.0):
     """Time the CPU oracle (scalar Python restatement of EM_solver) on a
     bounded sample: the same tree with the first k sites, one EM iteration
@@ -425,6 +440,7 @@ def main():
         out_ms, achieved, in_ms, inside_b / (in_ms * 1e-3) / 1e9))
 
     node_site = 2 * F.num_nodes * K  # over the whole batch
+    traffic = outside_traffic_from_profile(args) if B == 16 and args.workload == "em10k" else None
     res = {
         "metric": "EM iters/s at 10k cells x 300 sites",
         "value": round(em_per_s, 3),
@@ -448,7 +464,8 @@ def main():
         "node_site_evals_per_s": round(node_site * em_per_s, 1),
         "final_nllh_problem0": final_nll,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_source": PROFILE if traffic is not None else None,
                      "kernel": "outside pass (k_outside levels + k_root + k_rtilde)",
                      "algorithmic_bytes": outside_b, "ms": round(out_ms, 4),
                      "inside_pass": {"algorithmic_bytes": inside_b, "ms": round(in_ms, 4)}},
