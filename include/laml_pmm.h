@@ -75,6 +75,10 @@ typedef struct {
   const double* problem_phimin;/* [num_problems] phi cut-off 1/(K*ncells)   */
   const uint32_t* problem_flags; /* [num_problems] LAML_OPT_* bits          */
   double dmin, dmax;           /* branch-length box of the M-step           */
+  const double* site_qsum;     /* [sum_t K_t] total prior mass of each site's
+                                * packed alphabet, used where a missing ('?')
+                                * subtree sums over edit states; NULL = 1.0
+                                * (normalised priors, the reference EM model) */
 } laml_forest_desc;
 
 /* Host-only check of a description (shapes, topology, codes, offsets); no

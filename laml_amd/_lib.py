@@ -50,6 +50,7 @@ class ForestDesc(ctypes.Structure):
         ("problem_flags", _u32p),
         ("dmin", ctypes.c_double),
         ("dmax", ctypes.c_double),
+        ("site_qsum", _f64p),
     ]
 
 

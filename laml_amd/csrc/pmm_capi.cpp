@@ -463,6 +463,11 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   UP(prob_phimin, d->problem_phimin, P);
   UP(prob_nobs_phi, D.prob_nobs_phi.data(), P);
   UP(site_qoff, d->site_qoff, S + 1);
+  {
+    std::vector<double> qs(std::max(S, 1), 1.0);
+    if (d->site_qsum) std::copy(d->site_qsum, d->site_qsum + S, qs.begin());
+    UP(site_qsum, qs.data(), qs.size());
+  }
   UP(prior, d->prior, d->site_qoff[S]);
   UP(codes, codes_p.data(), leaf_rows);
 #undef UP

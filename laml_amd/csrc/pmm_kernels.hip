@@ -149,6 +149,7 @@ struct InNode {
   double2* beta;         // the node's inside row
   int32_t* bE;
   const int32_t* qoff;   // prior offsets of the tree's sites
+  const double* qsum;    // prior mass of the tree's sites
   double phi;
 };
 
@@ -166,6 +167,7 @@ __device__ __forceinline__ void in_node_load(const DevForest& F, int v, int t, i
   n.beta = F.beta + bv;
   n.bE = F.bE + bv;
   n.qoff = F.site_qoff + F.tree_site_base[t];
+  n.qsum = F.site_qsum + F.tree_site_base[t];
   n.phi = F.phi[p];
 }
 
@@ -180,8 +182,11 @@ __device__ __forceinline__ void inside_site(const DevForest& F, const InNode& n,
   load_child_rows(n.c[0], lf[0], k, n.phi, la, a0, aA, Ea, ca);
   load_child_rows(n.c[1], lf[1], k, n.phi, lb, b0, bA, Eb, cb);
   const int qo = row_at(n.qoff, k) - 1;
-  const double qa = la > 0 ? row_at(F.prior, (unsigned)(qo + la)) : 1.0;
-  const double qb = lb > 0 ? row_at(F.prior, (unsigned)(qo + lb)) : 1.0;
+  // a missing ('?') child may be in any edit state: its prior weight is the
+  // site's total prior mass
+  const double qs = row_at(n.qsum, k);
+  const double qa = la > 0 ? row_at(F.prior, (unsigned)(qo + la)) : qs;
+  const double qb = lb > 0 ? row_at(F.prior, (unsigned)(qo + lb)) : qs;
   const Eps ea = edge(la, a0, aA, Ea, qa, n.cf[0]);
   const Eps eb = edge(lb, b0, bA, Eb, qb, n.cf[1]);
   const double B0 = ea.e0 * eb.e0;
@@ -287,7 +292,7 @@ __device__ __forceinline__ void root_chunk(const DevForest& F, int t, int chunk,
     double b0, bA;
     load_child(F, r, leaf, k, phi, lab, b0, bA, E, code);
     const int qo = F.site_qoff[sb + k] - 1;
-    const double q = lab > 0 ? F.prior[qo + lab] : 1.0;
+    const double q = lab > 0 ? F.prior[qo + lab] : F.site_qsum[sb + k];
     const Eps er = edge(lab, b0, bA, E, q, cf);
     const int ex = exp_of(er.e0);
     const double Lm = ldexp(er.e0, -ex);
@@ -358,6 +363,7 @@ struct OutNode {
   const double* Lm;      // the tree's site likelihoods (mantissa, exponent)
   const int32_t* LE;
   const int32_t* qoff;
+  const double* qsum;
   double phi;
 };
 
@@ -383,6 +389,7 @@ __device__ __forceinline__ void out_node_load(const DevForest& F, int u, int t, 
   n.Lm = F.Lm + sb;
   n.LE = F.LE + sb;
   n.qoff = F.site_qoff + sb;
+  n.qsum = F.site_qsum + sb;
   n.phi = F.phi[p];
 }
 
@@ -400,13 +407,14 @@ __device__ __forceinline__ void outside_site(const DevForest& F, const OutNode& 
   const double iL = 1.0 / row_at(n.Lm, k);
   const int LE = row_at(n.LE, k);
   const int qo = row_at(n.qoff, k) - 1;
+  const double qs = row_at(n.qsum, k);
   int lab[2], E[2], code[2];
   double b0[2], bA[2], q[2];
   Eps ep[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     load_child_rows(n.c[i], lf[i], k, n.phi, lab[i], b0[i], bA[i], E[i], code[i]);
-    q[i] = lab[i] > 0 ? row_at(F.prior, (unsigned)(qo + lab[i])) : 1.0;
+    q[i] = lab[i] > 0 ? row_at(F.prior, (unsigned)(qo + lab[i])) : qs;
     ep[i] = edge(lab[i], b0[i], bA[i], E[i], q[i], n.cf[i]);
   }
 #pragma unroll
@@ -423,7 +431,8 @@ __device__ __forceinline__ void outside_site(const DevForest& F, const OutNode& 
                              : (lw > 0 ? fma(q[w], au.z, lw == gu ? au.w : 0.0) * ep[w].eA : 0.0);
     const int gg = lw < 0 ? gu : (lw > 0 ? lw : 0);
     const int ea = Eo + ep[w].e;
-    const double sumG = Gy + Gz;
+    // sum over edit states a of G(a) = q_a Gy + [a = gg] Gz
+    const double sumG = fma(qs, Gy, Gz);
     const double av0 = A * cf.p1n;
     const double base = A * cf.mut;
     const double Yv = fma(cf.pn, Gy, base);
@@ -459,7 +468,8 @@ __device__ __forceinline__ void outside_site(const DevForest& F, const OutNode& 
         const double f2 = ldexp(iL * cf.msil, min(ea + cf.esil - LE, 1000));
         acc[i][2] = fma(A, f2, acc[i][2]);
         acc[i][4] = fma(sumG, f2, acc[i][4]);
-        if (lf[i] && code[i] == -1) rt = fma(n.phi * (av0 + Yv + Zv), ldexp(iL, min(ea - LE, 1000)), rt);
+        if (lf[i] && code[i] == -1)
+          rt = fma(n.phi * (av0 + fma(qs, Yv, Zv)), ldexp(iL, min(ea - LE, 1000)), rt);
       }
     }
   }

@@ -49,6 +49,7 @@ struct DevForest {
   const double* prob_nobs_phi;  // [P] leaf entries observed as 0 or an edit
   // sites
   const int32_t* site_qoff;  // 1-based local state j -> prior[site_qoff[s] + j - 1]
+  const double* site_qsum;   // prior mass of each site's alphabet (1 for normalised priors)
   const double* prior;
   const int16_t* codes;  // leaf rows
   // per-call parameters

@@ -92,7 +92,13 @@ class SiteTable:
     """Per-site edit alphabets of one problem: state -> local index 1..m, and
     the normalised priors in CSR form."""
 
-    def __init__(self, Q):
+    def __init__(self, Q, normalized=True):
+        """Q: per-site dict state -> prior.  normalized: the priors of each
+        site sum to 1 over its alphabet (the reference EM model, which
+        normalises them, ML_solver.py:69-74); otherwise each site's total mass
+        is kept (``qsum``) and used where a missing subtree may be in any edit
+        state (the fastEM model, whose alphabets are the observed states only,
+        fast_em/phylogeny.py:62-94)."""
         self.K = len(Q)
         self.maps = []
         qoff = [0]
@@ -112,6 +118,10 @@ class SiteTable:
             qoff.append(len(prior))
         self.qoff = np.asarray(qoff, dtype=np.int32)
         self.prior = np.asarray(prior, dtype=np.float64)
+        if normalized:
+            self.qsum = np.ones(self.K)
+        else:
+            self.qsum = np.asarray([self.prior[self.qoff[k]:self.qoff[k + 1]].sum() for k in range(self.K)])
 
     def encode_row(self, row):
         out = np.empty(len(row), dtype=np.int16)
@@ -199,12 +209,15 @@ class ForestLayout:
         self.codes = np.concatenate(code_chunks).astype(np.int16) if code_chunks else np.zeros(0, np.int16)
         qoff_parts = []
         prior_parts = []
+        qsum_parts = []
         off = 0
         for p in problems:
             for _ in p.trees:
                 qoff_parts.append(p.sites.qoff[:-1].astype(np.int64) + off)
                 prior_parts.append(p.sites.prior)
+                qsum_parts.append(np.asarray(getattr(p.sites, "qsum", np.ones(p.sites.K)), dtype=np.float64))
                 off += len(p.sites.prior)
+        self.site_qsum = np.ascontiguousarray(np.concatenate(qsum_parts) if qsum_parts else np.zeros(0))
         self.site_qoff = np.concatenate(qoff_parts + [np.asarray([off], dtype=np.int64)]).astype(np.int32)
         self.prior = np.concatenate(prior_parts) if prior_parts else np.zeros(0)
         self.num_nodes = N
@@ -247,6 +260,7 @@ class ForestLayout:
         d.problem_flags = P(self.flags, ctypes.c_uint32)
         d.dmin = self.dmin
         d.dmax = self.dmax
+        d.site_qsum = P(self.site_qsum, ctypes.c_double)
         return d
 
     def validate(self):

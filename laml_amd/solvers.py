@@ -463,12 +463,79 @@ class EM_solver:
 
 
 class fastEM_solver(EM_solver):
-    """Surface of laml_libs.fastEM_solver.fastEM_solver (fastEM_solver.py:149-178),
-    evaluated by the exact FP64 device engine."""
+    """laml_libs.fastEM_solver.fastEM_solver (fastEM_solver.py:149-364).
+
+    The constructor takes the reference's extra inputs (data
+    'ordered_leaf_labels' and 'charMtrx_recode', prior 'Q_recode', as
+    runner.py:233-243 prepares them) or computes them with ``parse_tree`` /
+    ``parse_data``; ``score_tree`` runs the fastEM model through
+    ``laml_amd.fastem.EMOptimizer`` (exact FP64 EM on the device) with the
+    reference's start values and always-optimised nu and phi, and the exact
+    reference EM for 'final_optimization' (fastEM_solver.py:290-292)."""
     solver_name = "fastEM_solver"
 
     def __init__(self, treeList, data, prior, params={'nu': 0, 'phi': 0, 'sigma': 0}):
+        from .fastem import EMOptimizer, parse_data, parse_tree
+
         super().__init__(treeList, data, prior, params={'nu': 0, 'phi': 0, 'sigma': 0})
         self.solver_name = "fastEM_solver"
         self.data = data
         self.prior = prior
+        self.myEMOptimizers = []
+        self.ordered_leaf_labels = []
+        have = 'ordered_leaf_labels' in data and 'charMtrx_recode' in data and 'Q_recode' in prior
+        for tidx, tree in enumerate(self.trees):
+            if have:
+                labels = data['ordered_leaf_labels'][tidx]
+                C, Qr = data['charMtrx_recode'][tidx], prior['Q_recode'][tidx]
+            else:
+                parsed = parse_tree(tree)
+                phy = parse_data(parsed, {'charMtrx': self.charMtrx}, {'Q': self.Q})
+                labels = parsed['relabeling_vector'][:parsed['num_leaves']]
+                C, Qr = phy.character_matrix, phy.mutation_priors
+            self.ordered_leaf_labels.append(labels)
+            parse_tree(tree, has_branch_mask=False, ordered_leaf_labels=labels)
+            self.myEMOptimizers.append(EMOptimizer(Qr, C))
+
+    def score_tree(self, strategy={'ultra_constr': False, 'fixed_phi': None, 'fixed_nu': None, 'fixed_brlen': None,
+                                   'nodes_to_recompute': None}):
+        """fastEM_solver.score_tree (fastEM_solver.py:274-353).  Branches named
+        by ``nodes_to_recompute`` (anchor pairs) with positive length are
+        re-optimised, all others stay fixed when ``fixed_brlen`` is given
+        (the reference compares node labels to node objects there, so it
+        re-optimises none; this re-optimises the ones it names)."""
+        from .fastem import parse_tree
+
+        if 'final_optimization' in strategy:
+            return EM_solver.score_tree(self, strategy)
+        fixed_phi, fixed_nu = strategy['fixed_phi'], strategy['fixed_nu']
+        fixed_brlen = strategy['fixed_brlen']
+        nodes_to_recompute = strategy.get('nodes_to_recompute')
+        has_branch_mask = fixed_brlen is not None
+        results = []
+        for tidx, tree in enumerate(self.trees):
+            recompute = set()
+            if nodes_to_recompute:
+                recompute = {id(v) for v in find_LCAs(tree, list(nodes_to_recompute[tidx]))}
+            for node in tree.traverse_postorder():
+                node.mark_fixed = True
+                if fixed_brlen is None:
+                    continue
+                if id(node) in recompute and (node.edge_length or 0.0) > 0:
+                    node.mark_fixed = False
+            parsed = parse_tree(tree, has_branch_mask=has_branch_mask,
+                                ordered_leaf_labels=self.ordered_leaf_labels[tidx])
+            nu = fixed_nu if fixed_nu else 0.5
+            phi = fixed_phi if fixed_phi else 0.5
+            # the reference optimises nu and phi here whatever the strategy
+            # says (fastEM_solver.py:323-324)
+            out = self.myEMOptimizers[tidx].optimize(parsed, parsed['branch_lengths'], nu, phi,
+                                                     parsed['branch_mask'], False, False)
+            self.params.phi = out['phi']
+            self.params.nu = out['nu']
+            l2n = {v.label: v for v in tree.traverse_postorder()}
+            for i, label in enumerate(parsed['relabeling_vector']):
+                l2n[label].edge_length = float(out['branch_lengths'][i])
+            results = [-out['nllh']]
+        self._forest = None
+        return max(results), "optimal"
