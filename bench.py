@@ -225,6 +225,104 @@ def bench_serving(args):
     print(json.dumps(res), flush=True)
 
 
+QUARTET = {
+    # one NNI round of local candidate scoring on an optimised tree
+    "nni_local": dict(n=2000, K=200, m_lo=5, m_hi=50, alpha=1.0, nu=0.1, phi=0.15, seed=20261210, sample=16),
+}
+
+
+def bench_quartet(args):
+    """Local NNI scoring (nu, phi and all but the five branches around each
+    move fixed): every candidate of one round by quartet-local EM in one
+    launch, against the per-candidate solver (a device forest and EM per
+    candidate, as the reference builds a solver per candidate)."""
+    import random
+
+    import torch
+
+    from laml_amd import sim
+    from laml_amd.quartet import QuartetScorer
+    from laml_amd.solvers import DEFAULT_STRATEGY, EM_solver
+    from laml_amd.topology_search import Topology_search_batched, _nni_swap, _nni_unswap
+
+    c = QUARTET[args.workload]
+    tree, Q, codes, _ = sim.make_problem(c["n"], c["K"], c["m_lo"], c["m_hi"], c["alpha"], c["nu"], c["phi"],
+                                         c["seed"])
+    msa = sim.codes_to_charmtrx(tree, codes)
+    torch.cuda.set_device(0)
+    random.seed(c["seed"])
+    t0 = time.perf_counter()
+    s = EM_solver([tree.newick()], {"charMtrx": msa}, {"Q": Q}, {"nu": 0, "phi": 0})
+    s.score_tree(strategy=dict(DEFAULT_STRATEGY, fixed_brlen=None))
+    log("starting tree optimised in %.1f s (score %.4f)" % (time.perf_counter() - t0, -s.negative_llh()))
+    ts = Topology_search_batched(s.get_tree_newick(), EM_solver, data={"charMtrx": msa}, prior={"Q": Q},
+                                 params=s.get_params())
+    moves = ts._list_moves()
+    # warm-up round on a few moves, then the timed full round
+    sc = QuartetScorer(ts, DEFAULT_STRATEGY, EM_solver)
+    sc.score(moves[:8])
+    sc.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sc = QuartetScorer(ts, DEFAULT_STRATEGY, EM_solver)
+    t1 = time.perf_counter()
+    res = sc.score(moves)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    iters = sc.iters.copy()
+    sc.close()
+    q_time = t2 - t0
+    q_rate = len(moves) / q_time
+    log("quartet-local EM: %d candidates in %.3f s (set-up %.3f s, EM %.3f s), %.0f candidates/s, "
+        "mean %.1f EM iterations" % (len(moves), t2 - t0, t1 - t0, t2 - t1, q_rate, iters.mean()))
+    # per-candidate solver on a sample
+    sample = moves[:: max(1, len(moves) // c["sample"])][:c["sample"]]
+    idx = [moves.index(m) for m in sample]
+    worst = 0.0
+    times = []
+    for i, mv in zip(idx, sample):
+        u, v, uc, w = mv
+        st = ts._local_strategy(u, DEFAULT_STRATEGY)
+        _nni_swap(u, v, uc, w)
+        nwks = [t.newick() for t in ts.treeList_obj]
+        _nni_unswap(u, v, uc, w)
+        t0 = time.perf_counter()
+        sv = EM_solver(nwks, {"charMtrx": msa}, {"Q": Q}, ts.params)
+        nllh, _ = sv.optimize(initials=1, verbose=-1, fixed_phi=st["fixed_phi"], fixed_nu=st["fixed_nu"],
+                              fixed_brlen=st["fixed_brlen"], random_seeds=[7])
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        worst = max(worst, abs(-res[i][0] - nllh) / abs(nllh))
+    full_rate = 1.0 / float(np.mean(times))
+    log("per-candidate solver: %.1f ms per candidate (%d sampled), %.1f candidates/s; "
+        "max relative NLL difference to quartet EM %.2e" % (1e3 * np.mean(times), len(times), full_rate, worst))
+    if worst > 1e-9:
+        raise SystemExit("quartet-local EM differs from the per-candidate solver")
+    res_line = {
+        "metric": "NNI candidates scored/s (local branch optimisation)",
+        "value": round(q_rate, 1),
+        "unit": "candidates/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(q_time * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Yule tree, PMM-simulated characters)",
+        "config": {"workload": "one NNI round (%d candidates) on an optimised %d-leaf Yule tree x %d sites, "
+                               "M in [%d,%d], nu=%g, phi=%g; nu, phi and all but 5 branches fixed per candidate"
+                               % (len(moves), c["n"], c["K"], c["m_lo"], c["m_hi"], c["nu"], c["phi"]),
+                   "mean_em_iterations": round(float(iters.mean()), 2)},
+        "per_candidate_solver": {"candidates_per_s": round(full_rate, 2), "sampled": len(times),
+                                 "speedup_of_quartet": round(q_rate / full_rate, 1),
+                                 "max_rel_nll_diff": worst},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res_line), flush=True)
+
+
 def bench_scoring(args):
     """Batched scoring workload: one forest holding every candidate and
     independent tree, EM from a uniform start, against the plain path of one
@@ -336,7 +434,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="em10k", choices=sorted(WORKLOADS) + sorted(SCORING) + sorted(SERVING))
+    ap.add_argument("--workload", default="em10k", choices=sorted(WORKLOADS) + sorted(SCORING) + sorted(SERVING) + sorted(QUARTET))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-reps", type=int, default=10)
     ap.add_argument("--batch", type=int, default=16, help="independent problems per step")
@@ -355,6 +453,8 @@ def main():
         return bench_scoring(args)
     if args.workload in SERVING:
         return bench_serving(args)
+    if args.workload in QUARTET:
+        return bench_quartet(args)
 
     import torch
 

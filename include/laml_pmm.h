@@ -203,6 +203,28 @@ int laml_mstep(laml_forest* f, const double* counts, const double* rtilde,
 int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter, double tol,
             double* nll, int32_t* iters, double* trace, void* stream);
 
+/* Quartet-local EM of NNI candidates.  With nu, phi and every branch but
+ * the five around an NNI move fixed, the subtrees hanging off the move keep
+ * their inside vectors and the data above it its outside state, so each
+ * candidate's EM runs on those resident values alone, O(K) per EM step.
+ * Candidate c, after the move: v has children u and C, u has children A and
+ * W, p is v's parent and s v's sibling, all nodes of this forest's tree
+ * as it was before the move (p = s = -1 when v is a root).
+ * quartet_nodes: device int32 [5 * n] (p, s, A, W, C); lengths_in /
+ * lengths_out: device [5 * n] branch lengths above (A, W, u, C, v);
+ * optimise: device uint8 [n], bit j set = branch j is optimised; llh_const:
+ * device [n] log-likelihood of the problem's other trees; nu, phi: device
+ * [num_problems] (fixed); nll: device [n] final negative log-likelihood;
+ * iters: device int32 [n] with the numbering of laml_em.  Needs the inside
+ * and outside state of a preceding laml_estep on this forest with the same
+ * parameters (stream-ordered) and full inside storage.
+ * Replaces: the per-candidate solver build + EM_optimization of
+ * Topology_search.apply_nni / Topology_search_parallel.apply_nni with
+ * local_brlen_opt (Topology_search.py:242-322, Topology_search_parallel.py:57-63). */
+int laml_quartet_em(laml_forest* f, int32_t num_candidates, const int32_t* quartet_nodes, const double* lengths_in,
+                    const uint8_t* optimise, const double* llh_const, const double* nu, const double* phi,
+                    int32_t max_iter, double tol, double* lengths_out, double* nll, int32_t* iters, void* stream);
+
 /* Height-class model of the ultrametric constraint (see laml_amd/ultra.py).
  * Classes are sorted by (class level, problem); level_problem_off[l*P + p]
  * is the first class of problem p at class level l. */

@@ -1020,4 +1020,20 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
   return check_launch();
 }
 
+int laml_quartet_em(laml_forest* f, int32_t num_candidates, const int32_t* quartet_nodes, const double* lengths_in,
+                    const uint8_t* optimise, const double* llh_const, const double* nu, const double* phi,
+                    int32_t max_iter, double tol, double* lengths_out, double* nll, int32_t* iters, void* stream) {
+  if (!f || num_candidates < 0) return fail(LAML_EINVAL, "bad arguments");
+  if (num_candidates == 0) return LAML_OK;
+  if (!quartet_nodes || !lengths_in || !optimise || !llh_const || !nu || !phi || !lengths_out || !nll || !iters)
+    return fail(LAML_EINVAL, "null argument");
+  if (max_iter < 0) return fail(LAML_EINVAL, "max_iter < 0");
+  if (f->ckpt_L > 0) return fail(LAML_ESTATE, "quartet EM needs full inside storage (level checkpointing is on)");
+  DeviceGuard guard(f->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  laml::run_quartet_em(f->F, num_candidates, quartet_nodes, lengths_in, optimise, llh_const, nu, phi, max_iter, tol,
+                       lengths_out, nll, iters, s);
+  return check_launch();
+}
+
 }  // extern "C"

@@ -1241,6 +1241,296 @@ __global__ void __launch_bounds__(kUltraThreads) k_ultra(DevForest F, UltraDev U
 }
 
 // ---------------------------------------------------------------------------
+// Quartet-local EM of NNI candidates (Topology_search.py:242-322 with
+// local_brlen_opt): nu, phi and every branch except the five around the move
+// are fixed, so the subtrees hanging off the quartet (A and W below u, C below
+// v) keep their inside vectors and the data above v keeps its outside state.
+// A candidate's EM then needs, per site, only those inputs and the five free
+// branches: each EM step is O(K) instead of a pass over the tree.  One
+// wavefront per candidate; lanes stride over the tree's sites.
+//
+// Candidate c (after the move): v has children u and C, u has children A and
+// W; p is v's parent and s v's sibling (p = -1: v is the root and its branch
+// is the stem from the unedited ancestor).  Free branches in the order
+// (A, W, u, C, v); opt bit j set = branch j is optimised.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct QTop {            // data outside a child branch, at its parent's side
+  double A, Gy, Gz, GS, sumG;
+  int ea, Eo, gg;
+};
+
+struct QAlpha {          // outside state at a node: (o0, oS, Y, Z) 2^E, state g
+  double x, y, z, w;
+  int E, g;
+};
+
+__device__ __forceinline__ QTop q_top(const QAlpha& au, const Eps& ew, int lw, double qw, double qs) {
+  QTop t;
+  t.A = au.x * ew.e0;
+  t.Gy = lw < 0 ? au.z * ew.eA : 0.0;
+  t.Gz = lw < 0 ? au.w * ew.eA : (lw > 0 ? fma(qw, au.z, lw == au.g ? au.w : 0.0) * ew.eA : 0.0);
+  t.gg = lw < 0 ? au.g : (lw > 0 ? lw : 0);
+  t.ea = au.E + ew.e;
+  t.GS = lw < 0 ? au.y : 0.0;
+  t.sumG = fma(qs, t.Gy, t.Gz);
+  t.Eo = au.E;
+  return t;
+}
+
+__device__ __forceinline__ QAlpha q_child_alpha(const QTop& t, const Coef& cf) {
+  const double av0 = t.A * cf.p1n;
+  const double Yv = fma(cf.pn, t.Gy, t.A * cf.mut);
+  const double Zv = cf.pn * t.Gz;
+  const double T = ldexp(cf.msil * (t.A + t.sumG), cf.esil);
+  const double mx = fmax(fmax(av0, T), fmax(Yv, Zv));
+  const int tg = t.GS != 0.0 ? t.Eo + exp_of(t.GS) : kZeroTop;
+  int ec = max(mx != 0.0 ? t.ea + exp_of(mx) : kZeroTop, tg);
+  if (ec == kZeroTop) ec = t.Eo;
+  const double sa = ldexp(1.0, min(t.ea - ec, 1000));
+  const double sg = ldexp(1.0, min(t.Eo - ec, 1000));
+  QAlpha a;
+  a.x = av0 * sa;
+  a.y = fma(t.GS, sg, T * sa);
+  a.z = Yv * sa;
+  a.w = Zv * sa;
+  a.E = ec;
+  a.g = t.gg;
+  return a;
+}
+
+// posterior counts of the branch above a node with label lv and inside
+// (b0, bA) 2^E, prior q of its state (qs for '?'), given the top of the
+// branch and 1/L = iL 2^-LE
+__device__ __forceinline__ void q_post(const QTop& t, const Coef& cf, int lv, double b0, double bA, int E, double q,
+                                       double iL, int LE, double* S) {
+  if (lv == 0) {
+    S[0] += 1.0;
+    return;
+  }
+  const double f1 = ldexp(iL, min(t.ea + E - LE, 1000));
+  S[0] = fma(t.A * cf.p1n * b0, f1, S[0]);
+  S[1] = fma(t.A * cf.mut * (q * bA), f1, S[1]);
+  const double t3 = lv > 0 ? fma(q, t.Gy, lv == t.gg ? t.Gz : 0.0) : t.sumG;
+  S[3] = fma(t3 * (cf.pn * bA), f1, S[3]);
+  if (lv < 0) {
+    const double f2 = ldexp(iL * cf.msil, min(t.ea + cf.esil - LE, 1000));
+    S[2] = fma(t.A, f2, S[2]);
+    S[4] = fma(t.sumG, f2, S[4]);
+  }
+}
+
+__device__ __forceinline__ Coef q_coef(double d, double nu) {
+  Coef c;
+  c.p1n = exp(-(1.0 + nu) * d);
+  c.pn = exp(-nu * d);
+  c.mut = -expm1(-d) * c.pn;
+  int es;
+  c.msil = frexp(-expm1(-nu * d), &es);
+  c.esil = c.msil != 0.0 ? es : 0;
+  c.pad = 0;
+  return c;
+}
+
+// inside of a node from its two children's edge values; returns the label
+__device__ __forceinline__ int q_combine(int la, int lb, const Eps& ea, const Eps& eb, double& B0, double& BA,
+                                         int& E) {
+  const int lv = combine_label(la, lb);
+  const double b0 = ea.e0 * eb.e0;
+  const double bA = lv != 0 ? ea.eA * eb.eA : 0.0;
+  const int sh = min(-exp_of(fmax(b0, bA)), 1000);
+  const double sc = ldexp(1.0, sh);
+  B0 = b0 * sc;
+  BA = bA * sc;
+  E = ea.e + eb.e - sh;
+  return lv;
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_quartet_em(DevForest F, int ncand, const int32_t* __restrict__ qnodes,
+                                                    const double* __restrict__ len_in,
+                                                    const uint8_t* __restrict__ opt_mask,
+                                                    const double* __restrict__ llh_const,
+                                                    const double* __restrict__ nu_p,
+                                                    const double* __restrict__ phi_p, int max_iter, double tol,
+                                                    double* __restrict__ len_out, double* __restrict__ nll_out,
+                                                    int32_t* __restrict__ iters_out) {
+  const int c = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (c >= ncand) return;
+  const int lane = threadIdx.x & 63;
+  const int p = qnodes[c * 5 + 0], sib = qnodes[c * 5 + 1];
+  const int nA = qnodes[c * 5 + 2], nW = qnodes[c * 5 + 3], nC = qnodes[c * 5 + 4];
+  const int t = F.tree_of[nA];
+  const int prob = F.tree_prob[t];
+  const int K = F.tree_nsites[t];
+  const int sb = F.tree_site_base[t];
+  const double nu = nu_p[prob], phi = phi_p[prob];
+  const uint32_t opt = opt_mask[c];
+  double d[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) d[j] = len_in[c * 5 + j];
+  const ChildRows rA = child_rows(F, nA, nA < F.num_leaves);
+  const ChildRows rW = child_rows(F, nW, nW < F.num_leaves);
+  const ChildRows rC = child_rows(F, nC, nC < F.num_leaves);
+  const bool lfA = nA < F.num_leaves, lfW = nW < F.num_leaves, lfC = nC < F.num_leaves;
+  const bool top_root = p < 0;
+  Coef cs{};
+  ChildRows rS{};
+  bool lfS = false;
+  const double4* aP = nullptr;
+  const int32_t* aEP = nullptr;
+  const int16_t* aGP = nullptr;
+  if (!top_root) {
+    cs = F.coef[sib];
+    lfS = sib < F.num_leaves;
+    rS = child_rows(F, sib, lfS);
+    const int64_t rp = F.row_off[p];
+    aP = F.alpha + rp;
+    aEP = F.aE + rp;
+    aGP = F.aG + rp;
+  }
+  const int32_t* qoff = F.site_qoff + sb;
+  const double* qsum = F.site_qsum + sb;
+  const double kLn2 = 0.69314718055994530942;
+  double llh_prev = 0.0, llh = 0.0;
+  int iters = max_iter + 1;
+  for (int it = 0; it <= max_iter; ++it) {
+    Coef cf[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) cf[j] = q_coef(d[j], nu);
+    double S[5][5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) S[j][m] = 0.0;
+    double ll = 0.0;
+    for (int k = lane; k < K; k += 64) {
+      const int qo = row_at(qoff, (unsigned)k) - 1;
+      const double qs = row_at(qsum, (unsigned)k);
+      int labA, labW, labC, EA, EW, EC, code;
+      double b0A, bAA, b0W, bAW, b0C, bAC;
+      load_child_rows(rA, lfA, k, phi, labA, b0A, bAA, EA, code);
+      load_child_rows(rW, lfW, k, phi, labW, b0W, bAW, EW, code);
+      load_child_rows(rC, lfC, k, phi, labC, b0C, bAC, EC, code);
+      const double qA = labA > 0 ? row_at(F.prior, (unsigned)(qo + labA)) : qs;
+      const double qW = labW > 0 ? row_at(F.prior, (unsigned)(qo + labW)) : qs;
+      const double qC = labC > 0 ? row_at(F.prior, (unsigned)(qo + labC)) : qs;
+      const Eps eA = edge(labA, b0A, bAA, EA, qA, cf[0]);
+      const Eps eW = edge(labW, b0W, bAW, EW, qW, cf[1]);
+      double b0u, bAu;
+      int Eu;
+      const int labU = q_combine(labA, labW, eA, eW, b0u, bAu, Eu);
+      const double qU = labU > 0 ? row_at(F.prior, (unsigned)(qo + labU)) : qs;
+      const Eps eU = edge(labU, b0u, bAu, Eu, qU, cf[2]);
+      const Eps eC = edge(labC, b0C, bAC, EC, qC, cf[3]);
+      double b0v, bAv;
+      int Ev;
+      const int labV = q_combine(labU, labC, eU, eC, b0v, bAv, Ev);
+      const double qV = labV > 0 ? row_at(F.prior, (unsigned)(qo + labV)) : qs;
+      // top of v's branch: the unedited ancestor, or p's outside state with
+      // the sibling's edge values
+      QTop tv;
+      if (top_root) {
+        tv.A = 1.0;
+        tv.Gy = tv.Gz = tv.GS = tv.sumG = 0.0;
+        tv.ea = tv.Eo = tv.gg = 0;
+      } else {
+        int labS, ES;
+        double b0S, bAS;
+        load_child_rows(rS, lfS, k, phi, labS, b0S, bAS, ES, code);
+        const double qS = labS > 0 ? row_at(F.prior, (unsigned)(qo + labS)) : qs;
+        const Eps eS = edge(labS, b0S, bAS, ES, qS, cs);
+        const double4 a4 = row_at(aP, (unsigned)k);
+        QAlpha ap;
+        ap.x = a4.x;
+        ap.y = a4.y;
+        ap.z = a4.z;
+        ap.w = a4.w;
+        ap.E = row_at(aEP, (unsigned)k);
+        ap.g = row_at(aGP, (unsigned)k);
+        tv = q_top(ap, eS, labS, qS, qs);
+      }
+      const QAlpha av = q_child_alpha(tv, cf[4]);
+      // site likelihood from v's outside and inside states
+      double sumA;
+      if (labV > 0) sumA = fma(qV, av.z, labV == av.g ? av.w : 0.0);
+      else if (labV < 0) sumA = fma(qs, av.z, av.w);
+      else sumA = 0.0;
+      const double t1 = fma(av.x, b0v, sumA * bAv);
+      const double t2 = labV < 0 ? av.y : 0.0;
+      const int e1 = t1 != 0.0 ? Ev + exp_of(t1) : kZeroTop;
+      const int e2 = t2 != 0.0 ? exp_of(t2) : kZeroTop;
+      const int e = max(e1, e2);
+      double Lm = ldexp(t1, min(Ev - e, 1000)) + ldexp(t2, min(-e, 1000));
+      const int ex = exp_of(Lm);
+      Lm = ldexp(Lm, -ex);
+      const int LE = av.E + e + ex;
+      ll += log(Lm) + (double)LE * kLn2;
+      const double iL = 1.0 / Lm;
+      // posteriors: v's branch, then u and C below v, then A and W below u
+      q_post(tv, cf[4], labV, b0v, bAv, Ev, qV, iL, LE, S[4]);
+      const QTop tu = q_top(av, eC, labC, qC, qs);
+      q_post(tu, cf[2], labU, b0u, bAu, Eu, qU, iL, LE, S[2]);
+      const QTop tc = q_top(av, eU, labU, qU, qs);
+      q_post(tc, cf[3], labC, b0C, bAC, EC, qC, iL, LE, S[3]);
+      const QAlpha au = q_child_alpha(tu, cf[2]);
+      const QTop ta = q_top(au, eW, labW, qW, qs);
+      q_post(ta, cf[0], labA, b0A, bAA, EA, qA, iL, LE, S[0]);
+      const QTop tw = q_top(au, eA, labA, qA, qs);
+      q_post(tw, cf[1], labW, b0W, bAW, EW, qW, iL, LE, S[1]);
+    }
+    ll = seg_sum_last<64>(ll);
+    ll = __shfl(ll, 63, 64);
+    llh = ll + llh_const[c];
+    // EM_solver.py:483-512: stop when the relative change is below tol, or
+    // after max_iter M-steps (iteration count max_iter + 1)
+    if (it > 0 && tol > 0.0 && fabs((llh - llh_prev) / llh_prev) < tol) {
+      iters = it;
+      break;
+    }
+    if (it == max_iter) break;
+    llh_prev = llh;
+    // M-step of the free branches (EM_solver.py:339-366 for one branch)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      double sj[5];
+      double tot = 0.0;
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+        const double x = seg_sum_last<64>(S[j][m]);
+        sj[m] = fmax(kEpsS, __shfl(x, 63, 64));
+        tot += sj[m];
+      }
+      if (!((opt >> j) & 1u)) continue;
+#pragma unroll
+      for (int m = 0; m < 5; ++m) sj[m] = sj[m] / tot * (double)K;
+      BranchObj o;
+      o.nu = nu;
+      o.A = (nu + 1.0) * sj[0] + nu * sj[1] + nu * sj[3];
+      o.S1 = sj[1];
+      o.B = nu > kEpsNu ? (sj[2] + sj[4]) : 0.0;
+      d[j] = solve_branch(o, F.dmin, F.dmax, d[j]);
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) len_out[c * 5 + j] = d[j];
+    nll_out[c] = -llh;
+    iters_out[c] = iters;
+  }
+}
+
+void run_quartet_em(const DevForest& F, int ncand, const int32_t* qnodes, const double* len_in,
+                    const uint8_t* opt_mask, const double* llh_const, const double* nu, const double* phi,
+                    int max_iter, double tol, double* len_out, double* nll_out, int32_t* iters_out, hipStream_t s) {
+  if (ncand <= 0) return;
+  hipLaunchKernelGGL(k_quartet_em, dim3((ncand + 3) / 4), dim3(256), 0, s, F, ncand, qnodes, len_in, opt_mask,
+                     llh_const, nu, phi, max_iter, tol, len_out, nll_out, iters_out);
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 int seg_width(int max_k) {

@@ -417,6 +417,26 @@ class Forest(ForestLayout):
     def inside_rows(self):
         return int(self._lib.laml_forest_inside_rows(self.handle))
 
+    def quartet_em(self, qnodes, len_in, opt, llh_const, nu, phi, max_iter, tol):
+        """Quartet-local EM of NNI candidates on this forest's resident
+        inside/outside state (see laml_quartet_em).  qnodes [n, 5] (p, s, A,
+        W, C) level-order node ids; len_in [n, 5]; opt [n] bit masks;
+        llh_const [n].  Returns (lengths [n, 5], nll [n], iters [n]) device
+        tensors."""
+        dev = self.device
+        q = torch.as_tensor(np.ascontiguousarray(qnodes, dtype=np.int32), device=dev)
+        li = torch.as_tensor(np.ascontiguousarray(len_in, dtype=np.float64), device=dev)
+        om = torch.as_tensor(np.ascontiguousarray(opt, dtype=np.uint8), device=dev)
+        lc = torch.as_tensor(np.ascontiguousarray(llh_const, dtype=np.float64), device=dev)
+        n = int(q.shape[0])
+        lo = torch.empty((n, 5), dtype=torch.float64, device=dev)
+        nl = torch.empty(n, dtype=torch.float64, device=dev)
+        it = torch.empty(n, dtype=torch.int32, device=dev)
+        _lib.check(self._lib.laml_quartet_em(self.handle, n, _lib.ptr(q), _lib.ptr(li), _lib.ptr(om), _lib.ptr(lc),
+                                             _lib.ptr(nu), _lib.ptr(phi), int(max_iter), float(tol), _lib.ptr(lo),
+                                             _lib.ptr(nl), _lib.ptr(it), self.stream()))
+        return lo, nl, it
+
     def set_pass_mode(self, mode):
         """0 (levels) or 1 (tree walk: one launch per pass, a block per
         (tree, site chunk))."""

@@ -344,9 +344,46 @@ class Topology_search_batched(Topology_search):
     Topology_search_parallel.py:13-56, which scores batches of 16 candidates
     in a process pool)."""
 
-    def __init__(self, *args, batch_size=16, **kwargs):
+    def __init__(self, *args, batch_size=16, quartet=True, **kwargs):
         super().__init__(*args, **kwargs)
         self.batch_size = batch_size
+        self.quartet = quartet
+
+    def _list_moves(self, only_marked=False):
+        """The NNI moves of list_all_nni in the same order (and with the same
+        draws from the random generator), without writing tree texts."""
+        branches = self._nni_branches(only_marked)
+        _random.shuffle(branches)
+        moves = []
+        for u in branches:
+            v = u.get_parent()
+            w = next(x for x in v.child_nodes() if x is not u)
+            u_children = u.child_nodes()
+            _random.shuffle(u_children)
+            for u_child in u_children:
+                moves.append((u, v, u_child, w))
+        return moves
+
+    def _single_nni_quartet(self, curr_score, nni_iter, strategy, only_marked):
+        """Every candidate of the round scored by quartet-local EM in one
+        launch, then accepted in queue order (the queue order and the
+        acceptance rule of Topology_search_parallel.single_nni)."""
+        from .quartet import QuartetScorer
+        moves = self._list_moves(only_marked)
+        if not moves:
+            return curr_score, 0, False
+        scorer = QuartetScorer(self, strategy, self.solver)
+        try:
+            results = scorer.score(moves)
+            for move, (sc, lazy) in zip(moves, results):
+                if self.__accept_proposal__(curr_score, sc, nni_iter):
+                    res = lazy.materialise()
+                    _nni_swap(*move)
+                    self.update_from_solver(res)
+                    return sc, len(moves), True
+        finally:
+            scorer.close()
+        return curr_score, len(moves), False
 
     def list_all_nni(self, strategy, only_marked=False):
         """Every NNI move of the current trees as (newicks, strategy, move)
@@ -372,6 +409,9 @@ class Topology_search_batched(Topology_search):
         that did not reach status 'optimal' are queued once more with free
         branch lengths, and the first batch of that queue is tried after the
         main queue is exhausted."""
+        if (self.quartet and strategy['local_brlen_opt'] and not strategy['ultra_constr']
+                and getattr(self.solver, "solver_name", None) == "EM_solver"):
+            return self._single_nni_quartet(curr_score, nni_iter, strategy, only_marked)
         queue = self.list_all_nni(strategy, only_marked=only_marked)
         N = len(queue)
         retry = []
