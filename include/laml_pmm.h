@@ -203,6 +203,16 @@ int laml_mstep(laml_forest* f, const double* counts, const double* rtilde,
 int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter, double tol,
             double* nll, int32_t* iters, double* trace, void* stream);
 
+/* Posterior state probabilities of every (node, site) on the inside and
+ * outside state left by a preceding laml_estep with the same phi:
+ * post0 = P(node unedited | D), postS = P(node silenced | D), host double
+ * arrays in the layout of laml_forest_labels (leaf rows, then internal rows,
+ * K_t entries each).  Needs full inside storage.
+ * Replaces: the post0 / post1 of EM_solver.Estep_posterior
+ * (EM_solver.py:247-300) read by the annotation writer (runner.py:367-435). */
+int laml_posteriors(laml_forest* f, const double* phi, double* post0_host, double* postS_host, int64_t capacity,
+                    void* stream);
+
 /* Quartet-local EM of NNI candidates.  With nu, phi and every branch but
  * the five around an NNI move fixed, the subtrees hanging off the move keep
  * their inside vectors and the data above it its outside state, so each

@@ -1020,6 +1020,43 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
   return check_launch();
 }
 
+int laml_posteriors(laml_forest* f, const double* phi, double* post0_host, double* postS_host, int64_t capacity,
+                    void* stream) {
+  if (!f || !phi || !post0_host || !postS_host) return fail(LAML_EINVAL, "null argument");
+  const int64_t need = f->leaf_sites + f->int_sites;
+  if (capacity < need) return fail(LAML_EINVAL, "posterior capacity too small");
+  if (f->ckpt_L > 0) return fail(LAML_ESTATE, "posteriors need full inside storage (level checkpointing is on)");
+  DeviceGuard guard(f->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevForest F = f->F;
+  F.phi = phi;
+  const int64_t rows = f->leaf_rows + f->int_rows;
+  double *p0 = nullptr, *pS = nullptr;
+  LAML_HIP(hipMalloc(&p0, sizeof(double) * std::max<int64_t>(rows, 1)));
+  if (hipMalloc(&pS, sizeof(double) * std::max<int64_t>(rows, 1)) != hipSuccess) {
+    (void)hipFree(p0);
+    return fail(LAML_ENOMEM, "hipMalloc posteriors");
+  }
+  laml::run_node_post(F, f->W, f->leaf_rows, p0, pS, s);
+  std::vector<double> h0(rows), hS(rows);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h0.data(), p0, sizeof(double) * rows, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(hS.data(), pS, sizeof(double) * rows, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(p0);
+  (void)hipFree(pS);
+  if (e != hipSuccess) return fail(LAML_EDEVICE, std::string("posteriors: ") + hipGetErrorString(e));
+  const int N = f->F.num_nodes, n = f->F.num_leaves;
+  for (int v = 0; v < N; ++v) {
+    const int K = f->h_Kt[v];
+    const int64_t src = (v < n ? 0 : f->leaf_rows) + f->h_row_off[v];
+    const int64_t dst = (v < n ? 0 : f->leaf_sites) + f->h_desc_off[v];
+    std::memcpy(post0_host + dst, h0.data() + src, sizeof(double) * K);
+    std::memcpy(postS_host + dst, hS.data() + src, sizeof(double) * K);
+  }
+  return LAML_OK;
+}
+
 int laml_quartet_em(laml_forest* f, int32_t num_candidates, const int32_t* quartet_nodes, const double* lengths_in,
                     const uint8_t* optimise, const double* llh_const, const double* nu, const double* phi,
                     int32_t max_iter, double tol, double* lengths_out, double* nll, int32_t* iters, void* stream) {

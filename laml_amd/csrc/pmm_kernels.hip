@@ -1669,6 +1669,73 @@ void run_tree_walk(const DevForest& F, const TreeWalk& TW, int W, int phases, do
   });
 }
 
+// Posterior state probabilities of every (node, site): P(v = 0 | D) and
+// P(v = -1 | D), the post0 / post1 of EM_solver.Estep_posterior
+// (EM_solver.py:247-300, in linear space), from the resident inside and
+// outside state.  Segments index internal nodes; a leaf child's outside
+// state is formed from its parent's as in the outside pass.  Outputs use the
+// device row layout: internal node v at leaf_rows + row_off[v], leaf v at
+// row_off[v].
+template <int W>
+__global__ void __launch_bounds__(256) k_node_post(DevForest F, int64_t leaf_rows, double* __restrict__ p0,
+                                                   double* __restrict__ pS) {
+  Seg<W> g;
+  if (!g.init(F.num_leaves, F.num_nodes - F.num_leaves)) return;
+  const int u = g.v, k = g.k;
+  const int t = F.tree_of[u];
+  const int prob = F.tree_prob[t];
+  const int K = F.tree_nsites[t];
+  if (k >= K) return;
+  const int sb = F.tree_site_base[t];
+  const double phi = F.phi[prob];
+  const double iL = 1.0 / F.Lm[sb + k];
+  const int LE = F.LE[sb + k];
+  const int64_t ru = F.row_off[u];
+  const double4 a4 = F.alpha[ru + k];
+  QAlpha au;
+  au.x = a4.x;
+  au.y = a4.y;
+  au.z = a4.z;
+  au.w = a4.w;
+  au.E = F.aE[ru + k];
+  au.g = F.aG[ru + k];
+  const int labu = F.labels[ru + k];
+  const int64_t bu = F.brow[u];
+  const double2 bb = F.beta[bu + k];
+  const int Eu = F.bE[bu + k];
+  p0[leaf_rows + ru + k] = au.x * bb.x * ldexp(iL, min(au.E + Eu - LE, 1000));
+  pS[leaf_rows + ru + k] = labu < 0 ? au.y * ldexp(iL, min(au.E - LE, 1000)) : 0.0;
+  const int ch[2] = {F.child0[u], F.child1[u]};
+  const int qo = F.site_qoff[sb + k] - 1;
+  const double qs = F.site_qsum[sb + k];
+  int lab[2], E[2], code[2];
+  double b0[2], bA[2], q[2];
+  Eps ep[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    load_child(F, ch[i], ch[i] < F.num_leaves, k, phi, lab[i], b0[i], bA[i], E[i], code[i]);
+    q[i] = lab[i] > 0 ? F.prior[qo + lab[i]] : qs;
+    ep[i] = edge(lab[i], b0[i], bA[i], E[i], q[i], F.coef[ch[i]]);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    if (ch[i] >= F.num_leaves) continue;
+    const QTop tp = q_top(au, ep[1 - i], lab[1 - i], q[1 - i], qs);
+    const QAlpha av = q_child_alpha(tp, F.coef[ch[i]]);
+    const int64_t rv = F.row_off[ch[i]];
+    p0[rv + k] = av.x * b0[i] * ldexp(iL, min(av.E - LE, 1000));
+    pS[rv + k] = lab[i] < 0 ? av.y * ldexp(iL, min(av.E - LE, 1000)) : 0.0;
+  }
+}
+
+void run_node_post(const DevForest& F, int W, int64_t leaf_rows, double* p0, double* pS, hipStream_t s) {
+  const int cnt = F.num_nodes - F.num_leaves;
+  if (cnt <= 0) return;
+  LAML_DISPATCH_W(W, {
+    hipLaunchKernelGGL(k_node_post<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, leaf_rows, p0, pS);
+  });
+}
+
 void run_counts(const DevForest& F, double* counts, hipStream_t s) {
   hipLaunchKernelGGL(k_counts, dim3((F.num_nodes + 255) / 256), dim3(256), 0, s, F, counts);
 }

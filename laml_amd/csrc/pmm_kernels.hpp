@@ -145,6 +145,7 @@ void run_tree_walk(const DevForest& F, const TreeWalk& TW, int W, int phases, do
                    const int32_t* done, hipStream_t s);
 void run_llh(const DevForest& F, double* nll, EmState em, int max_iter, double tol, hipStream_t s);
 void run_em_reset(const EmState& em, int P, hipStream_t s);
+void run_node_post(const DevForest& F, int W, int64_t leaf_rows, double* p0, double* pS, hipStream_t s);
 void run_quartet_em(const DevForest& F, int ncand, const int32_t* qnodes, const double* len_in,
                     const uint8_t* opt_mask, const double* llh_const, const double* nu, const double* phi,
                     int max_iter, double tol, double* len_out, double* nll_out, int32_t* iters_out, hipStream_t s);

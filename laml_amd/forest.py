@@ -417,6 +417,20 @@ class Forest(ForestLayout):
     def inside_rows(self):
         return int(self._lib.laml_forest_inside_rows(self.handle))
 
+    def posteriors(self, phi):
+        """(post0, postS): P(node unedited | D) and P(node silenced | D) per
+        (node, site) on the state of the preceding estep, as (leaf rows,
+        internal rows) flat arrays in level order like ``labels``."""
+        K = self.tree_K[self.tree_of]
+        leaf_rows = int(K[: self.num_leaves].sum())
+        int_rows = int(K[self.num_leaves:].sum())
+        p0 = np.zeros(leaf_rows + int_rows)
+        pS = np.zeros(leaf_rows + int_rows)
+        f64 = ctypes.POINTER(ctypes.c_double)
+        _lib.check(self._lib.laml_posteriors(self.handle, _lib.ptr(phi), p0.ctypes.data_as(f64),
+                                             pS.ctypes.data_as(f64), p0.size, self.stream()))
+        return (p0[:leaf_rows], p0[leaf_rows:]), (pS[:leaf_rows], pS[leaf_rows:])
+
     def quartet_em(self, qnodes, len_in, opt, llh_const, nu, phi, max_iter, tol):
         """Quartet-local EM of NNI candidates on this forest's resident
         inside/outside state (see laml_quartet_em).  qnodes [n, 5] (p, s, A,
