@@ -207,6 +207,24 @@ class Tree:
                 par.children[idx:idx] = node.children
                 node.children = []
 
+    def extract_tree_without(self, labels, suppress_unifurcations=True):
+        """A copy of the tree without the leaves named in ``labels`` (internal
+        nodes left without children are removed too), unifurcations
+        suppressed (what dedup needs from treeswift's method of that name)."""
+        drop = set(labels)
+        t = read_tree_newick(self.newick())
+        t.is_rooted = self.is_rooted
+        for v in list(t.traverse_postorder()):
+            if v.is_root():
+                continue
+            if (v.is_leaf() and v.label in drop) or (not v.children and v.label is None):
+                par = v.parent
+                par.children.remove(v)
+                v.parent = None
+        if suppress_unifurcations:
+            t.suppress_unifurcations()
+        return t
+
     def height(self):
         """Root-to-tip path length maxima (excluding the stem)."""
         best = 0.0
