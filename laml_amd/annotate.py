@@ -62,8 +62,9 @@ def node_values_device(solver):
     out = []
     for ti, t in enumerate(solver.trees):
         res = {}
+        gids = solver._node_gids(F, ti)
         for pi, v in enumerate(t.traverse_postorder()):
-            g = F.inv[F.tree_base[ti] + pi]
+            g = gids[pi]
             if g < nl:
                 p0, pS = l0[g * K:(g + 1) * K], lS[g * K:(g + 1) * K]
             else:

@@ -88,6 +88,60 @@ def tree_arrays_from_tree(tree, codes_of_leaf, free_of_node=None, poly_of_node=N
     return TreeArrays(c0, c1, parent, length, free, codes, nodes, poly)
 
 
+def resolved_tree_arrays(tree, codes_of_leaf, free_of_node=None):
+    """Post-order arrays of a tree that may have polytomies, each resolved
+    for the device into a balanced binary tree of zero-length fixed
+    branches.  A zero-length branch has the identity transition (p = 1: no
+    edit, no silencing), so an added node's inside values are the products
+    of its children's, and the likelihood is that of the polytomy, whose
+    parent multiplies over all its children (ML_solver.py:174-179).
+    Returns (arrays, index of each node of ``tree`` in post-order)."""
+    c0l, c1l, lengths, frees, polys, rows = [], [], [], [], [], []
+
+    def add(a, b, length, free, poly):
+        c0l.append(a)
+        c1l.append(b)
+        lengths.append(length)
+        frees.append(free)
+        polys.append(poly)
+        return len(c0l) - 1
+
+    def join(ids):
+        if len(ids) == 1:
+            return ids[0]
+        h = len(ids) // 2
+        return add(join(ids[:h]), join(ids[h:]), 0.0, 0, True)
+
+    index = {}
+    post_index = []
+    for v in tree.traverse_postorder():
+        length = 0.0 if v.edge_length is None else float(v.edge_length)
+        free = 1 if free_of_node is None or free_of_node(v) else 0
+        if not v.children:
+            i = add(-1, -1, length, free, False)
+            rows.append(codes_of_leaf(v))
+        else:
+            ch = [index[id(c)] for c in v.children]
+            if len(ch) < 2:
+                raise ValueError("device forest requires internal nodes with at least two children")
+            h = len(ch) // 2
+            i = add(join(ch[:h]), join(ch[h:]), length, free, False)
+        index[id(v)] = i
+        post_index.append(i)
+    N = len(c0l)
+    if N < 3:
+        raise ValueError("device forest requires trees with at least two leaves")
+    c0 = np.asarray(c0l, dtype=np.int64)
+    c1 = np.asarray(c1l, dtype=np.int64)
+    parent = np.full(N, -1, dtype=np.int64)
+    inner = np.nonzero(c0 >= 0)[0]
+    parent[c0[inner]] = inner
+    parent[c1[inner]] = inner
+    codes = np.stack(rows).astype(np.int16)
+    ta = TreeArrays(c0, c1, parent, lengths, frees, codes, None, polys)
+    return ta, np.asarray(post_index, dtype=np.int64)
+
+
 class SiteTable:
     """Per-site edit alphabets of one problem: state -> local index 1..m, and
     the normalised priors in CSR form."""

@@ -13,6 +13,7 @@ but evaluates with the same exact FP64 engine (the reference's fastEM is an
 FP32 approximation with a different M-step, laml/fast_em/laml.py:62-110).
 """
 import random as _random
+import warnings
 from copy import deepcopy
 from math import isfinite
 
@@ -20,7 +21,7 @@ import numpy as np
 import torch
 
 from ._lib import LAML_OPT_NU, LAML_OPT_PHI
-from .forest import DMAX, DMIN, Forest, ProblemSpec, SiteTable, tree_arrays_from_tree
+from .forest import DMAX, DMIN, Forest, ProblemSpec, SiteTable, resolved_tree_arrays, tree_arrays_from_tree
 from .lca import find_LCAs
 from .tree import read_tree_newick
 from .ultra import HeightModel, UltraInfeasible, ultrametric_constr as _ultrametric_constr
@@ -53,8 +54,21 @@ class EM_solver:
     solver_name = "EM_solver"
 
     def __init__(self, treeList, data, prior, params={'nu': 0, 'phi': 0, 'sigma': 0}):
-        # ML_solver.__init__ (ML_solver.py:19-86)
+        self._read_inputs(treeList, data, prior, params)
         self.solver_name = "EM_solver"
+        # EM_solver.__init__ / __mark_polytomies__ (EM_solver.py:19-43)
+        self.has_polytomy = False
+        self.__mark_polytomies__(eps_len=self.dmin * 0.01)
+        self.num_edges = sum(sum(1 for _ in t.traverse_postorder()) for t in self.trees)
+        for t in self.trees:
+            for v in t.traverse_postorder():
+                if not hasattr(v, 'mark_fixed'):
+                    v.mark_fixed = False
+
+    def _read_inputs(self, treeList, data, prior, params):
+        """ML_solver.__init__ (ML_solver.py:19-86): trees (unifurcations
+        suppressed), character matrix (uninformative sites dropped when that
+        reconciles it with the priors), normalised priors, parameters."""
         charMtrx = data['charMtrx']
         Q = prior['Q']
         self.trees = []
@@ -84,14 +98,6 @@ class EM_solver:
         self.numsites = len(self.charMtrx[next(iter(self.charMtrx.keys()))])
         self.dmin = DMIN
         self.dmax = DMAX
-        # EM_solver.__init__ / __mark_polytomies__ (EM_solver.py:19-43)
-        self.has_polytomy = False
-        self.__mark_polytomies__(eps_len=self.dmin * 0.01)
-        self.num_edges = sum(sum(1 for _ in t.traverse_postorder()) for t in self.trees)
-        for t in self.trees:
-            for v in t.traverse_postorder():
-                if not hasattr(v, 'mark_fixed'):
-                    v.mark_fixed = False
         self._forest = None
         self._forest_key = None
         self._sites = None
@@ -155,6 +161,21 @@ class EM_solver:
             F.set_ultrametric(None)
             self._ultra_key = None
 
+    def _leaf_row(self, label):
+        """The character row a leaf is encoded from (EM_solver reads -1 as
+        silenced and '?' as missing, EM_solver.py:104-109)."""
+        return self.charMtrx[label]
+
+    def _tree_arrays(self, tree, codes_of, free_of, poly_of):
+        """Device arrays of one tree and, per node of the tree in post-order,
+        its index in them."""
+        ta = tree_arrays_from_tree(tree, codes_of, free_of, poly_of)
+        return ta, np.arange(ta.n_nodes)
+
+    def _node_gids(self, F, ti):
+        """Level-order device index of each node of tree ti (post-order)."""
+        return F.inv[F.tree_base[ti] + self._post_index[ti]]
+
     def _problem_spec(self, flags):
         sites = self._site_table()
         rows = {}
@@ -164,7 +185,7 @@ class EM_solver:
                 raise KeyError("leaf %r is not in the character matrix" % (v.label,))
             r = rows.get(v.label)
             if r is None:
-                r = sites.encode_row(self.charMtrx[v.label])
+                r = sites.encode_row(self._leaf_row(v.label))
                 rows[v.label] = r
             return r
 
@@ -174,7 +195,9 @@ class EM_solver:
         def poly_of(v):
             return bool(v.polytomy_mark)
 
-        arrays = [tree_arrays_from_tree(t, codes_of, free_of, poly_of) for t in self.trees]
+        built = [self._tree_arrays(t, codes_of, free_of, poly_of) for t in self.trees]
+        arrays = [a for a, _ in built]
+        self._post_index = [ix for _, ix in built]
         nobs = 0
         for t in self.trees:
             for v in t.traverse_leaves():
@@ -199,8 +222,12 @@ class EM_solver:
         return self._forest
 
     def _device_params(self, F):
-        lengths = [np.asarray([0.0 if v.edge_length is None else float(v.edge_length)
-                               for v in t.traverse_postorder()]) for t in self.trees]
+        lengths = []
+        for ti, t in enumerate(self.trees):
+            L = np.zeros(F.tree_base[ti + 1] - F.tree_base[ti])
+            L[self._post_index[ti]] = [0.0 if v.edge_length is None else float(v.edge_length)
+                                       for v in t.traverse_postorder()]
+            lengths.append(L)
         d = torch.tensor(F.lengths_to_level(lengths), dtype=torch.float64, device=F.device)
         nu = torch.tensor([float(self.params.nu)], dtype=torch.float64, device=F.device)
         phi = torch.tensor([float(self.params.phi)], dtype=torch.float64, device=F.device)
@@ -208,8 +235,8 @@ class EM_solver:
 
     def _store_params(self, F, d, nu, phi, only_free=True):
         per_tree = F.level_to_tree_lengths(d.cpu().numpy())
-        for t, L in zip(self.trees, per_tree):
-            for v, x in zip(t.traverse_postorder(), L):
+        for t, L, ix in zip(self.trees, per_tree, self._post_index):
+            for v, x in zip(t.traverse_postorder(), L[ix]):
                 if only_free and (v.polytomy_mark or getattr(v, 'mark_fixed', False)):
                     continue
                 v.edge_length = float(x)
@@ -231,13 +258,12 @@ class EM_solver:
         inv_maps = [{j: s for s, j in m.items()} for m in sites.maps]
         K = self.numsites
         out = []
-        lvl_of_post = F.inv
-        base = F.tree_base
         nl = F.num_leaves
         for ti, t in enumerate(self.trees):
             res = {}
+            gids = self._node_gids(F, ti)
             for pi, v in enumerate(t.traverse_postorder()):
-                g = lvl_of_post[base[ti] + pi]
+                g = gids[pi]
                 row = leaf_l[g * K:(g + 1) * K] if g < nl else int_l[(g - nl) * K:(g - nl + 1) * K]
                 res[v] = ['z' if x == 0 else ('?' if x < 0 else inv_maps[k][int(x)]) for k, x in enumerate(row)]
             out.append(res)
@@ -272,10 +298,10 @@ class EM_solver:
         d, nu, phi = self._device_params(F)
         counts, rt, nll = F.estep(d, nu, phi)
         c = counts.view(-1, 5).cpu().numpy()
-        base = F.tree_base
         for ti, t in enumerate(self.trees):
+            gids = self._node_gids(F, ti)
             for pi, v in enumerate(t.traverse_postorder()):
-                v.S_sum = c[F.inv[base[ti] + pi]].tolist()
+                v.S_sum = c[gids[pi]].tolist()
         self._last_estep = (d, nu, phi)
         return float(nll.item()), float(rt.item())
 
@@ -460,6 +486,173 @@ class EM_solver:
         st['_forest_key'] = None
         st.pop('_last_estep', None)
         return st
+
+
+class _DeviceObjective:
+    """SLSQP objective over x = [branch lengths (post-order over the trees),
+    nu, phi]: the device NLL and its gradient (one inside pass and one
+    outside pass per new x; the gradient call at the same x reuses them)."""
+
+    def __init__(self, solver, F, fixed_nu, fixed_phi):
+        self.F = F
+        self.ne = solver.num_edges
+        self.lvl = np.concatenate([solver._node_gids(F, ti) for ti in range(len(solver.trees))])
+        self.fixed_nu = fixed_nu
+        self.fixed_phi = fixed_phi
+        self.d_host = np.zeros(F.num_nodes)
+        self.d = torch.zeros(F.num_nodes, dtype=torch.float64, device=F.device)
+        self.nu = torch.zeros(1, dtype=torch.float64, device=F.device)
+        self.phi = torch.zeros(1, dtype=torch.float64, device=F.device)
+        self.x = None
+        self.evals = 0
+
+    def _eval(self, x):
+        x = np.asarray(x, dtype=np.float64)
+        if self.x is not None and np.array_equal(x, self.x):
+            return
+        ne = self.ne
+        self.d_host[self.lvl] = x[:ne]
+        self.d.copy_(torch.from_numpy(self.d_host))
+        self.nu.fill_(float(x[ne]) if self.fixed_nu is None else float(self.fixed_nu))
+        self.phi.fill_(float(x[ne + 1]) if self.fixed_phi is None else float(self.fixed_phi))
+        nll = self.F.nll(self.d, self.nu, self.phi)
+        gd, gnu, gphi = self.F.grad(self.d, self.nu, self.phi)
+        g = np.zeros(len(x))
+        g[:ne] = gd.cpu().numpy()[self.lvl]
+        if self.fixed_nu is None:
+            g[ne] = float(gnu.item())
+        if self.fixed_phi is None:
+            g[ne + 1] = float(gphi.item())
+        self.f = float(nll[0].item())
+        self.g = g
+        self.x = x.copy()
+        self.evals += 1
+
+    def fun(self, x):
+        self._eval(x)
+        return self.f
+
+    def jac(self, x):
+        self._eval(x)
+        return self.g
+
+
+class ML_solver(EM_solver):
+    """GPU-backed replacement of laml_libs.ML_solver.ML_solver
+    (ML_solver.py:18-377).
+
+    The trees stay as given, polytomies included (x and the tree text are
+    over their nodes); the device forest holds each polytomy resolved into
+    zero-length fixed branches, whose likelihood is the polytomy's.  A leaf
+    entry -1 is missing like '?' (ML_solver.az_partition labels it '?',
+    ML_solver.py:139-140).  optimize_one keeps the reference's SciPy SLSQP
+    call, bounds and linear constraints (ML_solver.py:338-377); its objective
+    is the device NLL and its gradient the device's exact gradient, where the
+    reference lets SLSQP difference the objective."""
+    solver_name = "ML_solver"
+
+    def __init__(self, treeList, data, prior, params={'nu': EPS, 'phi': EPS}):
+        self._read_inputs(treeList, data, prior, params)
+        self.solver_name = "ML_solver"
+        self.has_polytomy = any(len(v.children) > 2 for t in self.trees for v in t.traverse_postorder())
+        for t in self.trees:
+            for v in t.traverse_postorder():
+                v.polytomy_mark = False
+                v.mark_fixed = False
+
+    def _leaf_row(self, label):
+        return ['?' if x == -1 else x for x in self.charMtrx[label]]
+
+    def _tree_arrays(self, tree, codes_of, free_of, poly_of):
+        return resolved_tree_arrays(tree, codes_of, free_of)
+
+    def ultrametric_constr(self):
+        """ML_solver.ultrametric_constr (ML_solver.py:95-112): rows of M with
+        M x = 0 over x = [branch lengths, nu, phi] (binary trees)."""
+        N = len(self.ini_all())
+        M = []
+        idx = 0
+        cons = {}
+        for tree in self.trees:
+            for node in tree.traverse_postorder():
+                if node.is_leaf():
+                    c = [0.] * N
+                else:
+                    c1, c2 = node.children
+                    M.append([x - y for (x, y) in zip(cons[id(c1)], cons[id(c2)])])
+                    c = cons[id(c1)]
+                c[idx] = 1
+                cons[id(node)] = c
+                idx += 1
+        r0 = cons[id(self.trees[0].root)]
+        for tree in self.trees[1:]:
+            M.append([x - y for (x, y) in zip(r0, cons[id(tree.root)])])
+        return M
+
+    # ML_solver.py:207-222
+    def bound_nu(self, fixed_nu=None):
+        return (EPS, 10) if fixed_nu is None else (fixed_nu - EPS, fixed_nu + EPS)
+
+    def bound_phi(self, fixed_phi=None):
+        return (EPS, 0.99) if fixed_phi is None else (fixed_phi - EPS, fixed_phi + EPS)
+
+    def bound_brlen(self):
+        return [self.dmin] * self.num_edges, [self.dmax] * self.num_edges
+
+    def get_bound(self, keep_feasible=False, fixed_phi=None, fixed_nu=None):
+        from scipy import optimize
+
+        br_lower, br_upper = self.bound_brlen()
+        phi_lower, phi_upper = self.bound_phi(fixed_phi=fixed_phi)
+        nu_lower, nu_upper = self.bound_nu(fixed_nu=fixed_nu)
+        return optimize.Bounds(br_lower + [nu_lower, phi_lower], br_upper + [nu_upper, phi_upper],
+                               keep_feasible=keep_feasible)
+
+    def optimize_one(self, randseed, fixed_phi=None, fixed_nu=None, verbose=1, ultra_constr=False):
+        """ML_solver.optimize_one (ML_solver.py:338-377).  Returns (nllh,
+        "optimal") or (None, SLSQP's message)."""
+        from scipy import optimize
+        from scipy.sparse import csr_matrix
+
+        _random.seed(a=randseed)
+        x0 = self.ini_all(fixed_phi=fixed_phi, fixed_nu=fixed_nu)
+        self.az_partition()
+        bounds = self.get_bound(fixed_phi=fixed_phi, fixed_nu=fixed_nu)
+        constraints = []
+        A, b = [], []
+        idx = 0
+        for tree in self.trees:
+            for node in tree.traverse_postorder():
+                if node.mark_fixed:
+                    a = [0] * len(x0)
+                    a[idx] = 1
+                    A.append(a)
+                    b.append(node.edge_length)
+                idx += 1
+        if A:
+            constraints.append(optimize.LinearConstraint(csr_matrix(A), b, b, keep_feasible=False))
+        if ultra_constr:
+            M = self.ultrametric_constr()
+            constraints.append(optimize.LinearConstraint(csr_matrix(M), [0] * len(M), [0] * len(M),
+                                                         keep_feasible=False))
+        F = self._forest_for()
+        obj = _DeviceObjective(self, F, fixed_nu, fixed_phi)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            out = optimize.minimize(obj.fun, x0, jac=obj.jac, method="SLSQP",
+                                    options={'disp': verbose > 0, 'maxiter': 1000}, bounds=bounds,
+                                    constraints=constraints)
+        self.slsqp_evals = obj.evals
+        self.slsqp_iterations = int(out.nit)
+        if out.success:
+            self.x2params(out.x, fixed_phi=fixed_phi, fixed_nu=fixed_nu)
+            f = float(out.fun)
+        else:
+            if obj.x is not None:
+                self.x2params(obj.x, fixed_phi=fixed_phi, fixed_nu=fixed_nu)
+            f = None
+        status = "optimal" if out.success else out.message
+        return f, status
 
 
 class fastEM_solver(EM_solver):

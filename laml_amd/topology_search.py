@@ -275,12 +275,14 @@ def score_batch(jobs, data, prior, params, solver_cls=EM_solver, max_iter=1000, 
     ini_all, x2params) and becomes one problem of the forest; EM runs on all of
     them together with a per-problem convergence test (EM_solver.py:461-512).
     Returns a list of (solver, score, status).  Candidates with the
-    ultrametric constraint are scored one by one."""
+    ultrametric constraint, and every candidate of a solver other than
+    EM_solver (whose score_tree is not this EM), are scored one by one."""
     out = [None] * len(jobs)
     batch = []
+    batchable = getattr(solver_cls, "solver_name", None) == "EM_solver"
     for j, (nwks, st) in enumerate(jobs):
         s = solver_cls(nwks, data, prior, params)
-        if st.get('ultra_constr'):
+        if st.get('ultra_constr') or not batchable:
             sc, status = s.score_tree(strategy=st)
             out[j] = (s, sc, status)
             continue

@@ -1,0 +1,181 @@
+"""ML_solver on the device against the ML oracle, the reference's ML
+known-answer tests and its legacy cases: likelihood on trees with polytomies
+and -1 entries (relative 1e-9), the adjoint gradient against finite
+differences of the oracle, and SLSQP optima (free, fixed branches,
+ultrametric) against the oracle's SLSQP."""
+import math
+import random
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import golden_path
+from laml_amd.solvers import ML_solver, _DeviceObjective
+from oracle import ml_oracle
+from oracle.em_oracle import read_charmtrx, read_priors_csv
+from test_ml_oracle import ML_KATS, _kat, load_poly_case, random_multifurcating
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(trees, msa, Q, nu, phi):
+    return ML_solver(trees, {"charMtrx": msa}, {"Q": Q}, {"nu": nu, "phi": phi})
+
+
+@pytest.mark.parametrize("case", ML_KATS["ml_nllh"], ids=lambda c: c["name"])
+def test_ml_reference_constants(case):
+    T, msa, Q, nu, phi = _kat(case)
+    s = _solver([T], msa, Q, nu, phi)
+    s.az_partition()
+    got = s.negative_llh()
+    assert abs(got - case["nllh"]) < 2e-9
+    assert math.isclose(got, ml_oracle.negative_llh(T, msa, Q, nu, phi), rel_tol=1e-9, abs_tol=1e-12)
+
+
+def test_ml_legacy_polytomy_case():
+    T, msa, Q, nu, phi, want = load_poly_case()
+    s = _solver([T], msa, Q, nu, phi)
+    assert s.has_polytomy
+    assert math.isclose(s.negative_llh(), want, rel_tol=1e-9)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_ml_random_polytomies(seed):
+    rng = random.Random(400 + seed)
+    n_trees = 1 + seed % 2
+    trees = []
+    msa = None
+    Q = None
+    for _ in range(n_trees):
+        T, m, q = random_multifurcating(rng, 40, 24, 5, max_arity=2 + 3 * (seed % 3))
+        trees.append(T)
+        if msa is None:
+            msa, Q = m, q
+    nu, phi = [0.0, 0.2, 0.9][seed % 3], [0.0, 0.1, 0.3][(seed + 1) % 3]
+    s = _solver(trees, msa, Q, nu, phi)
+    want = ml_oracle.negative_llh(trees, msa, Q, nu, phi)
+    assert math.isclose(s.negative_llh(), want, rel_tol=1e-9)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_ml_gradient_matches_finite_differences(seed):
+    rng = random.Random(500 + seed)
+    T, msa, Q = random_multifurcating(rng, 16, 10, 4, max_arity=4)
+    s = _solver([T], msa, Q, 0.2, 0.1)
+    s.az_partition()
+    F = s._forest_for()
+    obj = _DeviceObjective(s, F, None, None)
+    x = np.asarray([v.edge_length for v in s.trees[0].traverse_postorder()] + [0.35, 0.12])
+    f = obj.fun(x)
+    g = obj.jac(x)
+    _, posts = ml_oracle.read_trees([T])
+    K = len(Q)
+    Qn = ml_oracle.normalise_Q(Q)
+    ml_oracle.az_partition(posts[0], msa, K)
+
+    def nll(y):
+        for v, d in zip(posts[0], y[:-2]):
+            v.edge_length = d
+        return -ml_oracle.lineage_llh(posts[0], Qn, y[-2], y[-1], K)
+
+    assert math.isclose(f, nll(x), rel_tol=1e-9)
+    h = 1e-6
+    for i in range(len(x)):
+        e = np.zeros(len(x))
+        e[i] = h
+        fd = (nll(x + e) - nll(x - e)) / (2 * h)
+        assert abs(g[i] - fd) <= 1e-6 * max(1.0, abs(fd)), (i, g[i], fd)
+
+
+def _ini_x0(s, seed, fixed_nu=None, fixed_phi=None):
+    random.seed(seed)
+    return s.ini_all(fixed_phi=fixed_phi, fixed_nu=fixed_nu)
+
+
+@pytest.mark.parametrize("nu,phi", [(1e-10, 1e-10), (1e-8, 0.3), (0.4, 1e-10), (3.0, 0.9)])
+def test_ml_likelihood_at_extreme_parameters(nu, phi):
+    # where nu * d is tiny the reference's 1 - p**nu cancels; the device is
+    # held to the exact (40-digit) value there
+    T, msa, Q = random_multifurcating(random.Random(650), 20, 12, 3, max_arity=4)
+    s = _solver([T], msa, Q, nu, phi)
+    want = ml_oracle.negative_llh_mp(T, msa, Q, nu, phi)
+    assert math.isclose(s.negative_llh(), want, rel_tol=1e-9)
+
+
+def _x_of(s):
+    return [v.edge_length for t in s.trees for v in t.traverse_postorder()] + [s.params.nu, s.params.phi]
+
+
+@pytest.mark.parametrize("poly", [False, True])
+def test_ml_optimize_one_matches_oracle_slsqp(poly):
+    """Each device SLSQP run ends at a local optimum of the reference's
+    objective (the oracle's SLSQP restarted there does not improve it), and
+    the best of several starts equals the oracle's best from the same starts.
+    Single runs may end in different local optima: the reference's
+    differenced gradient and the device's exact one take different paths
+    once a step lands on the nu/phi bounds."""
+    rng = random.Random(600 + poly)
+    T, msa, Q = random_multifurcating(rng, 10, 12, 3, max_arity=4 if poly else 2, p_sil=0.0)
+    dev, ora = [], []
+    for seed in range(5):
+        s = _solver([T], msa, Q, 0.1, 0.1)
+        x0 = _ini_x0(s, seed)
+        f, status = s.optimize_one(seed, verbose=-1)
+        assert status == "optimal"
+        # the solver holds the optimum it reports
+        assert math.isclose(s.negative_llh(), f, rel_tol=1e-12)
+        fr, _, _ = ml_oracle.slsqp_optimize([T], msa, Q, _x_of(s))
+        assert fr >= f - 1e-7 * abs(f)
+        fo, _, ok = ml_oracle.slsqp_optimize([T], msa, Q, x0)
+        assert ok
+        dev.append(f)
+        ora.append(fo)
+    assert math.isclose(min(dev), min(ora), rel_tol=1e-6)
+
+
+def test_ml_fixed_branches_and_ultrametric():
+    rng = random.Random(700)
+    T, msa, Q = random_multifurcating(rng, 8, 10, 3, max_arity=2, p_sil=0.0)
+    s = _solver([T], msa, Q, 0.1, 0.1)
+    nodes = list(s.trees[0].traverse_postorder())
+    # fix the branch above an internal node, through its leaf anchors
+    v = next(x for x in nodes if not x.is_leaf() and x.parent is not None)
+    below = [y.label for y in v.traverse_postorder() if y.is_leaf()]
+    fixed = [{(below[0], below[-1]): 0.4}]
+    nllh, status = s.optimize(initials=1, random_seeds=[5], fixed_brlen=fixed, ultra_constr=True, verbose=-1)
+    assert status == "optimal"
+    # the oracle from the same start point and constraints
+    s2 = _solver([T], msa, Q, 0.1, 0.1)
+    v2 = [x for x in s2.trees[0].traverse_postorder()][nodes.index(v)]
+    v2.edge_length = 0.4
+    v2.mark_fixed = True
+    x0 = _ini_x0(s2, 5)
+    fi = nodes.index(v)
+    fo, xo, ok = ml_oracle.slsqp_optimize([T], msa, Q, x0, fixed_idx=[fi], ultra=True)
+    assert ok
+    assert math.isclose(nllh, fo, rel_tol=1e-6)
+    # the optimum satisfies the constraints (checked on the returned tree text)
+    out = s.get_tree_newick()[0]
+    _, posts = ml_oracle.read_trees([out])
+    dist = {}
+    for y in posts[0]:
+        if y.is_leaf():
+            dist[id(y)] = 0.0
+        else:
+            ds = [dist[id(c)] + c.edge_length for c in y.children]
+            assert max(ds) - min(ds) < 1e-6
+            dist[id(y)] = ds[0]
+
+
+@pytest.mark.parametrize("case", ML_KATS["ml_optimize"], ids=lambda c: c["source"].split("/")[2])
+def test_ml_optimize_legacy_cases(case):
+    T = open(golden_path(case["tree_file"])).read().strip()
+    msa = read_charmtrx(golden_path(case["charMtrx_file"]))
+    K = len(next(iter(msa.values())))
+    Q = read_priors_csv(golden_path(case["prior_file"]), K)
+    s = _solver([T], msa, Q, 0.1, 0.1)
+    nllh, status = s.optimize(initials=1, random_seeds=[1], verbose=-1)
+    assert status == "optimal"
+    # one random start, as the reference's case (SLSQP's own tolerance)
+    assert math.isclose(nllh, case["nllh"], rel_tol=1e-5)
