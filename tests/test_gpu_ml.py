@@ -179,3 +179,39 @@ def test_ml_optimize_legacy_cases(case):
     assert status == "optimal"
     # one random start, as the reference's case (SLSQP's own tolerance)
     assert math.isclose(nllh, case["nllh"], rel_tol=1e-5)
+
+
+def test_ml_brute_force_optima_match_reference():
+    """The reference's topology tests compute their brute-force optima with
+    ML_solver.optimize(initials=1) over every rooted topology
+    (unit_tests_TopoSearch.py:51-62); the device ML_solver reaches the same
+    optima to 4 places."""
+    from helpers import load_topology_kats, rooted_topologies
+
+    Qt, MSA, BF, _ = load_topology_kats()
+    topos = rooted_topologies(["a", "b", "c", "d"])
+    assert len(topos) == 15
+    for key, msa_name, ultra in [("msa1", "msa1", False), ("msa2", "msa2", False), ("msa2_ultra", "msa2", True)]:
+        random.seed(3)
+        best = float("inf")
+        for T in topos:
+            s = ML_solver([T], {"charMtrx": MSA[msa_name]}, {"Q": Qt})
+            nllh, _ = s.optimize(initials=1, verbose=-1, ultra_constr=ultra)
+            best = min(best, nllh)
+        assert round(best - BF[key], 4) == 0, (key, best, BF[key])
+
+
+def test_ml_topology_search_reaches_reference_optimum():
+    """unit_tests_TopoSearch.py test_2: NNI search with ML_solver."""
+    from copy import deepcopy
+
+    from helpers import load_topology_kats
+    from laml_amd.solvers import DEFAULT_STRATEGY
+    from laml_amd.topology_search import Topology_search
+
+    Qt, MSA, BF, _ = load_topology_kats()
+    random.seed(11)
+    ts = Topology_search(["((a,c),(b,d));"], ML_solver, data={"charMtrx": MSA["msa1"]}, prior={"Q": Qt},
+                         params={"nu": 0, "phi": 0})
+    trees, score, params = ts.search(maxiter=200, nreps=1, strategy=deepcopy(DEFAULT_STRATEGY))
+    assert round(-score - BF["msa1"], 4) == 0, (-score, trees)
