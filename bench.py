@@ -38,7 +38,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(F):
+def algorithmic_bytes(F, fused=False):
     """Compulsory HBM bytes of one inside pass and one outside pass.
 
     Per (internal node, site): inside reads the node's label (2 B) and each
@@ -46,13 +46,17 @@ def algorithmic_bytes(F):
     exponent, 22 B) and writes beta + exponent (20 B).  Outside reads the
     node's outside state (alpha quad + exponent + specific state, 38 B), each
     child's inside state again (2 / 22 B) and writes each internal child's
-    outside state (38 B)."""
+    outside state (38 B).  With fused cherries (nodes whose children are both
+    leaves) a cherry has no rows: its parent reads the two leaf codes (4 B)
+    in place of its state, and the cherry's own steps move nothing else."""
     N, n = F.num_nodes, F.num_leaves
     K = F.tree_K[F.tree_of].astype(np.int64)
-    inter = np.arange(n, N)
+    lo, hi = (int(F.level_off[1]), int(F.level_off[2])) if fused else (n, n)
+    inter = np.arange(hi if fused else n, N)
     c = np.stack([F.child0[inter], F.child1[inter]])
-    rc = np.where(c < n, 2, 22)
-    wo = np.where(c < n, 0, 38)
+    cherry = (c >= lo) & (c < hi)
+    rc = np.where(c < n, 2, np.where(cherry, 4, 22))
+    wo = np.where((c < n) | cherry, 0, 38)
     Kv = K[inter]
     inside = int((Kv * (2 + 20 + rc.sum(0))).sum())
     outside = int((Kv * (38 + rc.sum(0) + wo.sum(0))).sum())
@@ -550,7 +554,7 @@ def main():
     # launches of k_outside, the fused top-level kernel, k_root, k_counts,
     # k_rtilde), timed with events on the launch stream over resident inside
     # state
-    inside_b, outside_b = algorithmic_bytes(F)
+    inside_b, outside_b = algorithmic_bytes(F, fused=F.fusion())
     F.nll(d, nu, phi)
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)

@@ -136,6 +136,21 @@ int64_t laml_forest_inside_rows(laml_forest* f);
 int laml_forest_set_pass_mode(laml_forest* f, int32_t mode);
 int32_t laml_forest_pass_mode(laml_forest* f);
 
+/* Cherry fusion (off by default; environment LAML_FUSE_CHERRIES=1 turns the
+ * default on; laml_amd.autotune tries it): in level mode with row kernels
+ * (more than 32 sites) and full inside storage, the nodes whose two children
+ * are leaves (the whole height-1 level) get no launch and no rows in the
+ * passes: their parents form their inside vectors from the leaf codes and run
+ * their posterior step in the outside pass.  Values are identical except for
+ * the order of floating-point sums over sites.  Needs every tree to have at
+ * least three leaves (otherwise it stays off).  laml_posteriors and
+ * laml_quartet_em read every node's rows, so they fail after a fused E-step:
+ * turn fusion off and rerun it.  laml_forest_fusion reports whether the next
+ * pass fuses (1) or not (0).
+ * No counterpart in the reference. */
+int laml_forest_set_fusion(laml_forest* f, int32_t on);
+int32_t laml_forest_fusion(laml_forest* f);
+
 int laml_forest_get_shapes(laml_forest* f, int32_t* inside_cpw, int32_t* outside_cpw, int32_t* top_level);
 int laml_forest_set_shapes(laml_forest* f, const int32_t* inside_cpw, const int32_t* outside_cpw,
                            int32_t top_level);

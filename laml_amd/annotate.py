@@ -52,6 +52,8 @@ def format_posterior(p0, p_minus_1, p_alpha, alpha, q):
 def node_values_device(solver):
     """Per tree: {node: (labels, post0 list, post1 list (log), S-sum list)}
     from one device E-step of ``solver`` (an EM_solver at its parameters)."""
+    # the posteriors read every node's rows: no cherry fusion in this E-step
+    solver._forest_for().set_fusion(False)
     solver.Estep()
     F = solver._forest
     d, nu, phi = solver._last_estep

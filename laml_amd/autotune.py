@@ -1,16 +1,18 @@
 """Launch-plan autotuner for a device forest.
 
 A forest's passes can run as level-synchronous launches (per level: the
-number of 64-site chunks one wavefront walks, and whether the small top
-levels share one fused launch) or as the tree walk (one launch per pass, a
-block per (tree, site chunk)).  Which is fastest depends on the forest: how
+number of 64-site chunks one wavefront walks, whether the small top levels
+share one fused launch, and whether the cherries -- nodes with two leaf
+children -- are fused into their parents) or as the tree walk (one launch per
+pass, a block per (tree, site chunk)).  Which is fastest depends on the forest: how
 many trees, how deep, how many sites, how uneven.  ``autotune`` times the
 E-step (inside pass, root, outside pass, counts) under candidate plans on
 the forest itself and keeps the fastest; every plan computes the same values
 up to the order of floating-point sums over sites.
 
-The search is a coarse pass over pass mode and top-level fusion, then a
-coordinate search over the chunks per wavefront of the largest levels.
+The search is a coarse pass over pass mode, top-level fusion and cherry
+fusion, then a coordinate search over the chunks per wavefront of the largest
+levels.
 """
 import numpy as np
 import torch
@@ -33,29 +35,39 @@ def _time_estep(F, d, nu, phi, reps):
 
 
 class Plan:
-    """A launch plan: pass mode, per-level chunks per wavefront, top level."""
+    """A launch plan: pass mode, per-level chunks per wavefront, top level,
+    cherry fusion."""
 
-    def __init__(self, mode, inside_cpw, outside_cpw, top_level):
+    def __init__(self, mode, inside_cpw, outside_cpw, top_level, fuse=True):
         self.mode = int(mode)
         self.inside_cpw = np.asarray(inside_cpw, dtype=np.int32).copy()
         self.outside_cpw = np.asarray(outside_cpw, dtype=np.int32).copy()
         self.top_level = int(top_level)
+        self.fuse = bool(fuse)
+
+    def copy(self):
+        return Plan(self.mode, self.inside_cpw, self.outside_cpw, self.top_level, self.fuse)
 
     def apply(self, F):
         F.set_pass_mode(PASS_LEVELS)
         F.set_shapes(self.inside_cpw, self.outside_cpw, self.top_level)
+        F.set_fusion(self.fuse)
         F.set_pass_mode(self.mode)
 
     def describe(self):
         if self.mode == PASS_TREES:
             return {"mode": "trees"}
-        return {"mode": "levels", "top_level": self.top_level,
+        return {"mode": "levels", "top_level": self.top_level, "cherry_fusion": self.fuse,
                 "inside_cpw": self.inside_cpw.tolist(), "outside_cpw": self.outside_cpw.tolist()}
 
 
 def current_plan(F):
     a, b, top = F.get_shapes()
-    return Plan(F.pass_mode(), a, b, top)
+    mode = F.pass_mode()
+    F.set_pass_mode(PASS_LEVELS)
+    fuse = F.fusion()
+    F.set_pass_mode(mode)
+    return Plan(mode, a, b, top, fuse)
 
 
 def autotune(F, d, nu, phi, reps=3, max_levels=8, verbose=False):
@@ -75,8 +87,19 @@ def autotune(F, d, nu, phi, reps=3, max_levels=8, verbose=False):
 
     best = base
     best_ms = measure(base, "default")
-    cands = [("no fused top", Plan(PASS_LEVELS, base.inside_cpw, base.outside_cpw, H)),
-             ("tree walk", Plan(PASS_TREES, base.inside_cpw, base.outside_cpw, base.top_level))]
+    cands = [("no fused top", Plan(PASS_LEVELS, base.inside_cpw, base.outside_cpw, H, base.fuse)),
+             ("tree walk", Plan(PASS_TREES, base.inside_cpw, base.outside_cpw, base.top_level, base.fuse))]
+    # cherry fusion, where the forest allows it (it replaces the fused top
+    # kernels: the top levels run as row launches)
+    mode = F.pass_mode()
+    F.set_pass_mode(PASS_LEVELS)
+    F.set_fusion(True)
+    can_fuse = F.fusion()
+    F.set_fusion(base.fuse)
+    F.set_pass_mode(mode)
+    if can_fuse:
+        cands.append(("cherry fusion" if not base.fuse else "no cherry fusion",
+                      Plan(PASS_LEVELS, base.inside_cpw, base.outside_cpw, H, not base.fuse)))
     for tag, plan in cands:
         ms = measure(plan, tag)
         if ms < best_ms:
@@ -84,7 +107,8 @@ def autotune(F, d, nu, phi, reps=3, max_levels=8, verbose=False):
     rows = base.inside_cpw[1:].max() > 0  # row kernels in use (more than 32 sites)
     if best.mode == PASS_LEVELS and rows:
         sizes = np.diff(F.level_off)
-        big = [h for h in np.argsort(-sizes) if 1 <= h < min(H, best.top_level)][:max_levels]
+        first = 2 if best.fuse else 1  # the cherry level has no launch of its own when fused
+        big = [h for h in np.argsort(-sizes) if first <= h < min(H, best.top_level)][:max_levels]
         max_chunks = int(max(best.inside_cpw.max(), best.outside_cpw.max(), 1))
         options = sorted({1, 2, max(1, max_chunks // 2), max_chunks})
         for which in ("outside_cpw", "inside_cpw"):
@@ -93,7 +117,7 @@ def autotune(F, d, nu, phi, reps=3, max_levels=8, verbose=False):
                 for c in options:
                     if c == cur:
                         continue
-                    trial = Plan(best.mode, best.inside_cpw, best.outside_cpw, best.top_level)
+                    trial = best.copy()
                     getattr(trial, which)[h] = c
                     ms = measure(trial, "%s[%d]=%d" % (which, h, c))
                     if ms < best_ms * 0.995:

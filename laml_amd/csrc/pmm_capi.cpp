@@ -71,6 +71,14 @@ struct laml_forest {
   std::vector<int32_t> in_cpw, out_cpw;
   // pass mode: 0 = level-synchronous launches, 1 = tree walk (one launch)
   int pass_mode = 0;
+  // cherry fusion (the height-1 level formed and consumed by its parents):
+  // requested (off by default; the autotuner tries it), possible for this
+  // forest (row kernels, every tree with at least three leaves), and
+  // whether the last inside pass ran with it
+  bool fuse = false;
+  bool cherry_ok = false;
+  bool inside_fused = false;
+  laml::CherrySplit split;  // per-level node lists of the fused passes
   laml::TreeWalk TW{};
   // host copies for re-layout of the inside vectors
   std::vector<int32_t> h_parent;
@@ -401,6 +409,40 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   F.dmax = d->dmax;
   F.chunk_w = f->W;
   F.max_chunks = (max_k + f->W - 1) / f->W;
+  F.cherry_end = d->num_leaves;
+  {
+    // the height-1 level holds exactly the nodes whose children are both
+    // leaves; fusing it needs every tree root above it and row kernels
+    bool ok = d->num_levels >= 3 && f->W == 64;
+    for (int t = 0; ok && t < d->num_trees; ++t) ok = d->tree_root[t] >= d->level_off[2];
+    f->cherry_ok = ok;
+    const char* env = getenv("LAML_FUSE_CHERRIES");
+    f->fuse = env && atoi(env) != 0;
+    if (ok) {
+      // per level from 2 up: nodes without a cherry child, then with one
+      const int H = d->num_levels;
+      const int c_beg = d->level_off[1], c_end = d->level_off[2];
+      auto is_cherry = [&](int c) { return c >= c_beg && c < c_end; };
+      std::vector<int32_t> list;
+      list.reserve(d->num_nodes - c_end);
+      f->split.off.assign(2 * H + 1, 0);
+      for (int h = 2; h < H; ++h) {
+        f->split.off[2 * h] = (int32_t)list.size();
+        for (int v = d->level_off[h]; v < d->level_off[h + 1]; ++v)
+          if (!is_cherry(d->child0[v]) && !is_cherry(d->child1[v])) list.push_back(v);
+        f->split.off[2 * h + 1] = (int32_t)list.size();
+        for (int v = d->level_off[h]; v < d->level_off[h + 1]; ++v)
+          if (is_cherry(d->child0[v]) || is_cherry(d->child1[v])) list.push_back(v);
+      }
+      f->split.off[2 * H] = (int32_t)list.size();
+      for (int h = 0; h < 2; ++h) f->split.off[2 * h] = f->split.off[2 * h + 1] = 0;
+      int rc2 = f->upload(&f->split.nodes, list.data(), list.size());
+      if (rc2) {
+        delete f;
+        return rc2;
+      }
+    }
+  }
   f->in_cpw = laml::default_cpw(f->level_off, f->W, F.max_chunks);
   f->out_cpw = f->in_cpw;
   // tree-walk plan: each tree's per-level node ranges and its (tree, chunk)
@@ -727,6 +769,16 @@ int laml_forest_set_pass_mode(laml_forest* f, int32_t mode) {
 
 int32_t laml_forest_pass_mode(laml_forest* f) { return f ? f->pass_mode : -1; }
 
+int laml_forest_set_fusion(laml_forest* f, int32_t on) {
+  if (!f) return fail(LAML_EINVAL, "null forest");
+  f->fuse = on != 0;
+  f->plan_version++;
+  return LAML_OK;
+}
+
+static bool fuse_now(const laml_forest* f);
+int32_t laml_forest_fusion(laml_forest* f) { return f ? (fuse_now(f) ? 1 : 0) : -1; }
+
 int laml_forest_set_shapes(laml_forest* f, const int32_t* inside_cpw, const int32_t* outside_cpw, int32_t top_level) {
   if (!f) return fail(LAML_EINVAL, "null forest");
   const int H = (int)f->level_off.size() - 1;
@@ -845,19 +897,42 @@ static int check_launch() {
   return LAML_OK;
 }
 
+// whether the next inside pass fuses the cherries (level launches with full
+// inside storage only)
+static bool fuse_now(const laml_forest* f) {
+  return f->fuse && f->cherry_ok && f->pass_mode == LAML_PASS_LEVELS && f->ckpt_L == 0 && f->W == 64;
+}
+
+static DevForest with_cherries(const laml_forest* f, const DevForest& F, bool fused) {
+  DevForest G = F;
+  G.cherry_end = fused ? f->level_off[2] : F.num_leaves;
+  return G;
+}
+
 // inside pass and root step (site likelihoods) in the forest's pass mode
 static void pass_inside_root(laml_forest* f, const DevForest& F, double* site_llh, const int32_t* done,
                              hipStream_t s) {
   if (f->pass_mode == 1) {
+    f->inside_fused = false;
     laml::run_tree_walk(F, f->TW, f->W, 3, site_llh, done, s);
   } else {
-    laml::run_inside(F, f->level_off, f->W, f->h_top, f->in_cpw, done, s);
-    laml::run_root(F, f->W, site_llh, done, s);
+    f->inside_fused = fuse_now(f);
+    const DevForest G = with_cherries(f, F, f->inside_fused);
+    laml::run_inside(G, f->level_off, f->W, f->h_top, f->in_cpw, done, s, f->inside_fused ? &f->split : nullptr);
+    laml::run_root(G, f->W, site_llh, done, s);
   }
 }
 
-// outside pass with posterior partials (and the root step first when asked)
+// outside pass with posterior partials (and the root step first when asked);
+// after an inside pass with fused cherries the outside pass fuses them too
+// (the cherries' inside rows were not written)
 static void pass_outside(laml_forest* f, const DevForest& F, bool with_root, const int32_t* done, hipStream_t s) {
+  if (f->inside_fused) {
+    const DevForest G = with_cherries(f, F, true);
+    if (with_root) laml::run_root(G, f->W, nullptr, done, s);
+    laml::run_outside(G, f->level_off, f->W, f->h_top, f->out_cpw, done, s, &f->split);
+    return;
+  }
   if (f->pass_mode == 1) {
     laml::run_tree_walk(F, f->TW, f->W, (with_root ? 2 : 0) | 4, nullptr, done, s);
     return;
@@ -892,6 +967,7 @@ int laml_estep(laml_forest* f, const double* d, const double* nu, const double* 
   F.phi = phi;
   laml::run_coef(F, d, nu, nullptr, s);
   if (f->pass_mode == 1) {
+    f->inside_fused = false;
     laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, nullptr, s);
   } else {
     pass_inside_root(f, F, nullptr, nullptr, s);
@@ -946,6 +1022,7 @@ static void em_step(laml_forest* f, const DevForest& F, const EmState& em, doubl
   if (f->pass_mode == 1 && !last) {
     // one launch for the whole E-step; the outside half of a problem that
     // converges at this step is not used (its M-step is skipped)
+    f->inside_fused = false;
     laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, em.done, s);
     laml::run_llh(F, nll, em, max_iter, tol, s);
   } else {
@@ -1026,6 +1103,9 @@ int laml_posteriors(laml_forest* f, const double* phi, double* post0_host, doubl
   const int64_t need = f->leaf_sites + f->int_sites;
   if (capacity < need) return fail(LAML_EINVAL, "posterior capacity too small");
   if (f->ckpt_L > 0) return fail(LAML_ESTATE, "posteriors need full inside storage (level checkpointing is on)");
+  if (f->inside_fused)
+    return fail(LAML_ESTATE, "posteriors need every node's rows: rerun the E-step with cherry fusion off "
+                             "(laml_forest_set_fusion)");
   DeviceGuard guard(f->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
   DevForest F = f->F;
@@ -1066,6 +1146,9 @@ int laml_quartet_em(laml_forest* f, int32_t num_candidates, const int32_t* quart
     return fail(LAML_EINVAL, "null argument");
   if (max_iter < 0) return fail(LAML_EINVAL, "max_iter < 0");
   if (f->ckpt_L > 0) return fail(LAML_ESTATE, "quartet EM needs full inside storage (level checkpointing is on)");
+  if (f->inside_fused)
+    return fail(LAML_ESTATE, "quartet EM needs every node's rows: rerun the E-step with cherry fusion off "
+                             "(laml_forest_set_fusion)");
   DeviceGuard guard(f->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
   laml::run_quartet_em(f->F, num_candidates, quartet_nodes, lengths_in, optimise, llh_const, nu, phi, max_iter, tol,

@@ -97,38 +97,106 @@ __device__ __forceinline__ const T& row_at(const T* p, unsigned k) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + k * (unsigned)sizeof(T));
 }
 
-// Rows of one child node at its parent's sites: leaf codes, or label +
-// inside vector.  Held as pointers to the row start so that, where the node is
-// wave-uniform, every per-site access is a scalar base plus a 32-bit lane
-// offset.
+// Kinds of child a pass kernel meets.  A *cherry* is an internal node whose
+// children are both leaves; when the pass fuses cherries (F.cherry_end >
+// F.num_leaves: nodes [num_leaves, cherry_end) are exactly the cherries, the
+// whole height-1 level) no cherry row is stored: its parent forms the
+// cherry's inside vector from the two leaf codes, and in the outside pass
+// runs the cherry's posterior step itself, so the height-1 level costs no
+// launch and no row traffic.
+// Template kind arguments: a fixed kind, or kNoCherry (leaf or internal,
+// read at run time: kernels that never meet a fused cherry carry no cherry
+// code).
+constexpr int kInternal = 0, kLeaf = 1, kCherry = 2, kNoCherry = -2;
+
+__device__ __forceinline__ int child_kind(const DevForest& F, int c) {
+  return c < F.num_leaves ? kLeaf : (c < F.cherry_end ? kCherry : kInternal);
+}
+
+template <int KT>
+__device__ __forceinline__ int resolve_kind(int k) {
+  return KT >= 0 ? KT : (KT == kNoCherry ? (k == kLeaf ? kLeaf : kInternal) : k);
+}
+
+// Rows of one child node at its parent's sites: leaf codes, label + inside
+// vector, or (cherry) the codes and branch coefficients of its two leaves.
+// Held as pointers to the row start so that, where the node is wave-uniform,
+// every per-site access is a scalar base plus a 32-bit lane offset.
 struct ChildRows {
-  const int16_t* code;   // leaf codes (leaf) or az-partition labels (internal)
+  const int16_t* code;   // leaf codes (leaf), az-partition labels (internal), first leaf's codes (cherry)
   const double2* beta;   // inside mantissas (internal)
   const int32_t* bE;     // inside exponent (internal)
+  const int16_t* code2;  // second leaf's codes (cherry)
+  Coef g[2];             // the cherry's leaf branches
+  int gc[2];             // the cherry's leaves
 };
 
-__device__ __forceinline__ ChildRows child_rows(const DevForest& F, int c, bool leaf) {
+__device__ __forceinline__ ChildRows child_rows(const DevForest& F, int c, int kind) {
   ChildRows r;
-  const int64_t ro = F.row_off[c];
-  if (leaf) {
-    r.code = F.codes + ro;
-    r.beta = nullptr;
-    r.bE = nullptr;
+  r.beta = nullptr;
+  r.bE = nullptr;
+  r.code2 = nullptr;
+  r.gc[0] = r.gc[1] = -1;
+  if (kind == kLeaf) {
+    r.code = F.codes + F.row_off[c];
+  } else if (kind == kCherry) {
+    const int a = F.child0[c], b = F.child1[c];
+    r.code = F.codes + F.row_off[a];
+    r.code2 = F.codes + F.row_off[b];
+    r.g[0] = F.coef[a];
+    r.g[1] = F.coef[b];
+    r.gc[0] = a;
+    r.gc[1] = b;
   } else {
     const int64_t br = F.brow[c];
-    r.code = F.labels + ro;
+    r.code = F.labels + F.row_off[c];
     r.beta = F.beta + br;
     r.bE = F.bE + br;
   }
   return r;
 }
+__device__ __forceinline__ ChildRows child_rows(const DevForest& F, int c, bool leaf) {
+  return child_rows(F, c, leaf ? kLeaf : kInternal);
+}
 
-__device__ __forceinline__ void load_child_rows(const ChildRows& r, bool leaf, int k, double phi, int& lab,
+// inside vector of a leaf at site k, seen from its parent over its branch
+__device__ __forceinline__ Eps leaf_edge(int code, double phi, const double* __restrict__ prior, int qo, double qs,
+                                         const Coef& cf, int& lab) {
+  double b0, bA;
+  leaf_beta(code, phi, lab, b0, bA);
+  const double q = lab > 0 ? row_at(prior, (unsigned)(qo + lab)) : qs;
+  return edge(lab, b0, bA, 0, q, cf);
+}
+
+// The inside vector a cherry's node would store at site k: the same
+// operations as the inside pass for a node with two leaf children.
+__device__ __forceinline__ void cherry_beta(const ChildRows& r, int k, double phi, const double* __restrict__ prior,
+                                            int qo, double qs, int& lab, double& b0, double& bA, int& E) {
+  int la, lb;
+  const Eps ea = leaf_edge(row_at(r.code, k), phi, prior, qo, qs, r.g[0], la);
+  const Eps eb = leaf_edge(row_at(r.code2, k), phi, prior, qo, qs, r.g[1], lb);
+  lab = combine_label(la, lb);
+  const double B0 = ea.e0 * eb.e0;
+  const double BA = lab != 0 ? ea.eA * eb.eA : 0.0;
+  const int sh = min(-exp_of(fmax(B0, BA)), 1000);
+  const double sc = ldexp(1.0, sh);
+  b0 = B0 * sc;
+  bA = BA * sc;
+  E = ea.e + eb.e - sh;
+}
+
+// inside state of a child at site k (qo, qs: the site's prior offset and
+// mass, used for a cherry's leaves)
+__device__ __forceinline__ void load_child_rows(const ChildRows& r, int kind, int k, double phi,
+                                                const double* __restrict__ prior, int qo, double qs, int& lab,
                                                 double& b0, double& bA, int& E, int& code) {
-  if (leaf) {
+  if (kind == kLeaf) {
     code = row_at(r.code, k);
     leaf_beta(code, phi, lab, b0, bA);
     E = 0;
+  } else if (kind == kCherry) {
+    code = 0;
+    cherry_beta(r, k, phi, prior, qo, qs, lab, b0, bA, E);
   } else {
     code = 0;
     lab = row_at(r.code, k);
@@ -138,11 +206,15 @@ __device__ __forceinline__ void load_child_rows(const ChildRows& r, bool leaf, i
     E = row_at(r.bE, k);
   }
 }
+__device__ __forceinline__ void load_child_rows(const ChildRows& r, bool leaf, int k, double phi, int& lab,
+                                                double& b0, double& bA, int& E, int& code) {
+  load_child_rows(r, leaf ? kLeaf : kInternal, k, phi, nullptr, 0, 0.0, lab, b0, bA, E, code);
+}
 
 // per-node values of the inside pass (uniform over the sites of the node)
 struct InNode {
   int K;
-  bool lf[2];
+  int kind[2];
   Coef cf[2];
   ChildRows c[2];
   const int16_t* lab;    // the node's labels
@@ -153,14 +225,17 @@ struct InNode {
   double phi;
 };
 
+// CH: the node may have a fused cherry child (otherwise children are leaves
+// or stored internal nodes, and no cherry code is generated)
+template <bool CH>
 __device__ __forceinline__ void in_node_load(const DevForest& F, int v, int t, int p, int K, InNode& n) {
   n.K = K;
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int c = i == 0 ? F.child0[v] : F.child1[v];
-    n.lf[i] = c < F.num_leaves;
+    n.kind[i] = CH ? child_kind(F, c) : (c < F.num_leaves ? kLeaf : kInternal);
     n.cf[i] = F.coef[c];
-    n.c[i] = child_rows(F, c, n.lf[i]);
+    n.c[i] = child_rows(F, c, n.kind[i]);
   }
   n.lab = F.labels + F.row_off[v];
   const int64_t bv = F.brow[v];
@@ -171,32 +246,52 @@ __device__ __forceinline__ void in_node_load(const DevForest& F, int v, int t, i
   n.phi = F.phi[p];
 }
 
-// inside vector (below the node) of internal node n at site k.  LEAF0 /
-// LEAF1: 1 = that child is a leaf, 0 = internal, -1 = read n.lf at run time.
-template <int LEAF0, int LEAF1>
-__device__ __forceinline__ void inside_site(const DevForest& F, const InNode& n, int k) {
-  const bool lf[2] = {LEAF0 >= 0 ? LEAF0 == 1 : n.lf[0], LEAF1 >= 0 ? LEAF1 == 1 : n.lf[1]};
-  const int lv = row_at(n.lab, k);
-  int la, lb, Ea, Eb, ca, cb;
-  double a0, aA, b0, bA;
-  load_child_rows(n.c[0], lf[0], k, n.phi, la, a0, aA, Ea, ca);
-  load_child_rows(n.c[1], lf[1], k, n.phi, lb, b0, bA, Eb, cb);
-  const int qo = row_at(n.qoff, k) - 1;
+// The inside step of one site in three phases: row loads, prior gathers
+// (they need the loaded labels), then arithmetic and stores.
+struct InSite {
+  int lv, qo, la, lb, Ea, Eb, ca, cb;
+  double qs, a0, aA, b0, bA, qa, qb;
+};
+
+template <int K0, int K1>
+__device__ __forceinline__ void inside_load(const DevForest& F, const InNode& n, int k, InSite& x) {
+  const int kind[2] = {resolve_kind<K0>(n.kind[0]), resolve_kind<K1>(n.kind[1])};
+  x.lv = row_at(n.lab, k);
+  x.qo = row_at(n.qoff, k) - 1;
   // a missing ('?') child may be in any edit state: its prior weight is the
   // site's total prior mass
-  const double qs = row_at(n.qsum, k);
-  const double qa = la > 0 ? row_at(F.prior, (unsigned)(qo + la)) : qs;
-  const double qb = lb > 0 ? row_at(F.prior, (unsigned)(qo + lb)) : qs;
-  const Eps ea = edge(la, a0, aA, Ea, qa, n.cf[0]);
-  const Eps eb = edge(lb, b0, bA, Eb, qb, n.cf[1]);
+  x.qs = row_at(n.qsum, k);
+  load_child_rows(n.c[0], kind[0], k, n.phi, F.prior, x.qo, x.qs, x.la, x.a0, x.aA, x.Ea, x.ca);
+  load_child_rows(n.c[1], kind[1], k, n.phi, F.prior, x.qo, x.qs, x.lb, x.b0, x.bA, x.Eb, x.cb);
+}
+
+__device__ __forceinline__ void inside_gather(const DevForest& F, InSite& x) {
+  x.qa = x.la > 0 ? row_at(F.prior, (unsigned)(x.qo + x.la)) : x.qs;
+  x.qb = x.lb > 0 ? row_at(F.prior, (unsigned)(x.qo + x.lb)) : x.qs;
+}
+
+__device__ __forceinline__ void inside_finish(const InNode& n, int k, const InSite& x) {
+  const Eps ea = edge(x.la, x.a0, x.aA, x.Ea, x.qa, n.cf[0]);
+  const Eps eb = edge(x.lb, x.b0, x.bA, x.Eb, x.qb, n.cf[1]);
   const double B0 = ea.e0 * eb.e0;
-  const double BA = lv != 0 ? ea.eA * eb.eA : 0.0;
+  const double BA = x.lv != 0 ? ea.eA * eb.eA : 0.0;
   // renormalise the pair by a power of two (bounded so that a denormal pair
   // cannot produce an infinite scale)
   const int sh = min(-exp_of(fmax(B0, BA)), 1000);
   const double sc = ldexp(1.0, sh);
   row_at(n.beta, k) = make_double2(B0 * sc, BA * sc);
   row_at(n.bE, k) = ea.e + eb.e - sh;
+}
+
+// inside vector (below the node) of internal node n at site k.  K0 / K1: the
+// kind of each child (kLeaf, kInternal, kCherry), or kNoCherry to read
+// n.kind at run time.
+template <int K0, int K1>
+__device__ __forceinline__ void inside_site(const DevForest& F, const InNode& n, int k) {
+  InSite x;
+  inside_load<K0, K1>(F, n, k, x);
+  inside_gather(F, x);
+  inside_finish(n, k, x);
 }
 
 // segment form: one site k of node v
@@ -207,8 +302,8 @@ __device__ __forceinline__ void inside_node(const DevForest& F, int v, int k, co
   if (k >= K) return;
   if (done && done[p]) return;
   InNode n;
-  in_node_load(F, v, t, p, K, n);
-  inside_site<-1, -1>(F, n, k);
+  in_node_load<false>(F, v, t, p, K, n);
+  inside_site<kNoCherry, kNoCherry>(F, n, k);
 }
 
 template <int W>
@@ -220,21 +315,30 @@ __global__ void __launch_bounds__(256) k_inside(DevForest F, int beg, int cnt, c
 
 // Row form (K > 32): one wavefront per node walks chunks [c0, c0 + cpw) of 64
 // sites.  The node number is wave-uniform, so the node's metadata are scalar
-// loads and the leaf tests scalar branches.
-template <int LEAF0, int LEAF1>
+// loads and the child-kind tests scalar branches.
+template <int K0, int K1>
 __device__ __forceinline__ void inside_row_run(const DevForest& F, const InNode& n, int c0, int c1, int lane) {
   for (int c = c0; c < c1; ++c) {
     const int k = c * 64 + lane;
-    if (k < n.K) inside_site<LEAF0, LEAF1>(F, n, k);
+    if (k < n.K) inside_site<K0, K1>(F, n, k);
   }
 }
 
-__global__ void __launch_bounds__(256) k_inside_rows(DevForest F, int beg, int cnt, int cpw,
-                                                      const int32_t* __restrict__ done) {
+// The wave's node: entry beg + wv of a node list, or of the level range.
+// Returns -1 past the end.
+__device__ __forceinline__ int row_node(int beg, int cnt, const int32_t* __restrict__ nodes) {
   const int wv = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-  if (wv >= cnt) return;
+  if (wv >= cnt) return -1;
+  return nodes ? __builtin_amdgcn_readfirstlane(nodes[beg + wv]) : beg + wv;
+}
+
+// Nodes whose children are leaves or stored internal nodes.
+__global__ void __launch_bounds__(256) k_inside_rows(DevForest F, int beg, int cnt, int cpw,
+                                                      const int32_t* __restrict__ done,
+                                                      const int32_t* __restrict__ nodes) {
+  const int v = row_node(beg, cnt, nodes);
+  if (v < 0) return;
   const int lane = threadIdx.x & 63;
-  const int v = beg + wv;
   const int t = F.tree_of[v];
   const int p = F.tree_prob[t];
   const int K = F.tree_nsites[t];
@@ -244,13 +348,42 @@ __global__ void __launch_bounds__(256) k_inside_rows(DevForest F, int beg, int c
   if (done && done[p]) return;
   const int c1 = min(c0 + cpw, nch);
   InNode n;
-  in_node_load(F, v, t, p, K, n);
-  if (n.lf[0]) {
-    if (n.lf[1]) inside_row_run<1, 1>(F, n, c0, c1, lane);
-    else inside_row_run<1, 0>(F, n, c0, c1, lane);
+  in_node_load<false>(F, v, t, p, K, n);
+  if (n.kind[0] == kLeaf) {
+    if (n.kind[1] == kLeaf) inside_row_run<kLeaf, kLeaf>(F, n, c0, c1, lane);
+    else inside_row_run<kLeaf, kInternal>(F, n, c0, c1, lane);
   } else {
-    if (n.lf[1]) inside_row_run<0, 1>(F, n, c0, c1, lane);
-    else inside_row_run<0, 0>(F, n, c0, c1, lane);
+    if (n.kind[1] == kLeaf) inside_row_run<kInternal, kLeaf>(F, n, c0, c1, lane);
+    else inside_row_run<kInternal, kInternal>(F, n, c0, c1, lane);
+  }
+}
+
+// Nodes with at least one fused cherry child (a separate kernel, so that the
+// cherry code does not raise the register count of the common case).
+__global__ void __launch_bounds__(256) k_inside_rows_ch(DevForest F, int beg, int cnt, int cpw,
+                                                         const int32_t* __restrict__ done,
+                                                         const int32_t* __restrict__ nodes) {
+  const int v = row_node(beg, cnt, nodes);
+  if (v < 0) return;
+  const int lane = threadIdx.x & 63;
+  const int t = F.tree_of[v];
+  const int p = F.tree_prob[t];
+  const int K = F.tree_nsites[t];
+  const int nch = (K + 63) >> 6;
+  const int c0 = (int)blockIdx.y * cpw;
+  if (c0 >= nch) return;
+  if (done && done[p]) return;
+  const int c1 = min(c0 + cpw, nch);
+  InNode n;
+  in_node_load<true>(F, v, t, p, K, n);
+  if (n.kind[0] == kCherry) {
+    if (n.kind[1] == kCherry) inside_row_run<kCherry, kCherry>(F, n, c0, c1, lane);
+    else if (n.kind[1] == kLeaf) inside_row_run<kCherry, kLeaf>(F, n, c0, c1, lane);
+    else inside_row_run<kCherry, kInternal>(F, n, c0, c1, lane);
+  } else if (n.kind[0] == kLeaf) {
+    inside_row_run<kLeaf, kCherry>(F, n, c0, c1, lane);
+  } else {
+    inside_row_run<kInternal, kCherry>(F, n, c0, c1, lane);
   }
 }
 
@@ -351,7 +484,7 @@ __global__ void __launch_bounds__(256) k_root(DevForest F, double* __restrict__ 
 struct OutNode {
   int K;
   int ch[2];
-  bool lf[2];
+  int kind[2];
   Coef cf[2];
   ChildRows c[2];
   double4* ca[2];        // children's outside rows (internal children)
@@ -367,15 +500,16 @@ struct OutNode {
   double phi;
 };
 
+template <bool CH>
 __device__ __forceinline__ void out_node_load(const DevForest& F, int u, int t, int p, int K, OutNode& n) {
   n.K = K;
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int c = i == 0 ? F.child0[u] : F.child1[u];
     n.ch[i] = c;
-    n.lf[i] = c < F.num_leaves;
+    n.kind[i] = CH ? child_kind(F, c) : (c < F.num_leaves ? kLeaf : kInternal);
     n.cf[i] = F.coef[c];
-    n.c[i] = child_rows(F, c, n.lf[i]);
+    n.c[i] = child_rows(F, c, n.kind[i]);
     const int64_t rc = F.row_off[c];
     n.ca[i] = F.alpha + rc;
     n.caE[i] = F.aE + rc;
@@ -393,35 +527,24 @@ __device__ __forceinline__ void out_node_load(const DevForest& F, int u, int t, 
   n.phi = F.phi[p];
 }
 
-// One site k of node n: the children's outside states and the posterior
-// counts of both child branches, added to acc; the dropout responsibility of
-// missing leaf children added to rt.  LEAF0 / LEAF1: 1 = that child is a leaf,
-// 0 = internal, -1 = read n.lf at run time.
-template <int LEAF0, int LEAF1>
-__device__ __forceinline__ void outside_site(const DevForest& F, const OutNode& n, int k, double (&acc)[2][5],
-                                             double& rt) {
-  const bool lf[2] = {LEAF0 >= 0 ? LEAF0 == 1 : n.lf[0], LEAF1 >= 0 ? LEAF1 == 1 : n.lf[1]};
-  const double4 au = row_at(n.a, k);
-  const int Eo = row_at(n.aE, k);
-  const int gu = row_at(n.aG, k);
-  const double iL = 1.0 / row_at(n.Lm, k);
-  const int LE = row_at(n.LE, k);
-  const int qo = row_at(n.qoff, k) - 1;
-  const double qs = row_at(n.qsum, k);
-  int lab[2], E[2], code[2];
-  double b0[2], bA[2], q[2];
-  Eps ep[2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    load_child_rows(n.c[i], lf[i], k, n.phi, lab[i], b0[i], bA[i], E[i], code[i]);
-    q[i] = lab[i] > 0 ? row_at(F.prior, (unsigned)(qo + lab[i])) : qs;
-    ep[i] = edge(lab[i], b0[i], bA[i], E[i], q[i], n.cf[i]);
-  }
+// The posterior step of one node at one site.  In: the node's outside state
+// au 2^Eo with specific state gu, 1/L 2^-LE of the site, and for both
+// children their inside state (lab, b0, bA, E), leaf code, prior q, edge
+// values ep and branch coefficients cf.  Adds the counts of both child
+// branches to acc and the dropout responsibility of missing leaf children to
+// rt, and hands every non-leaf child's outside state to sink(i, quad, ec, gg)
+// (the quad carries 2^ec; gg is the specific state).
+template <typename Sink>
+__device__ __forceinline__ void post_site(const double4& au, int Eo, int gu, double iL, int LE, double qs,
+                                          double phi, const int (&lab)[2], const double (&b0)[2],
+                                          const double (&bA)[2], const int (&E)[2], const int (&code)[2],
+                                          const double (&q)[2], const Eps (&ep)[2], const Coef (&cfs)[2],
+                                          const bool (&lf)[2], double (&acc)[2][5], double& rt, Sink&& sink) {
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int w = 1 - i;
     const int lw = lab[w];
-    const Coef& cf = n.cf[i];
+    const Coef& cf = cfs[i];
     // values below share the exponent ea = Eo + e_w:
     // A = alpha_u(0) eps_w(0) (joint of the data outside v with u = 0);
     // G(a) = alpha_u(a) eps_w(a) = q_a Gy + [a = gg] Gz
@@ -450,9 +573,7 @@ __device__ __forceinline__ void outside_site(const DevForest& F, const OutNode& 
       if (ec == kZeroTop) ec = Eo;
       const double sa = ldexp(1.0, min(ea - ec, 1000));
       const double sg = ldexp(1.0, min(Eo - ec, 1000));
-      row_at(n.ca[i], k) = make_double4(av0 * sa, fma(GS, sg, T * sa), Yv * sa, Zv * sa);
-      row_at(n.caE[i], k) = ec;
-      row_at(n.caG[i], k) = (int16_t)gg;
+      sink(i, make_double4(av0 * sa, fma(GS, sg, T * sa), Yv * sa, Zv * sa), ec, gg);
     }
     const int lv = lab[i];
     if (lv == 0) {
@@ -469,9 +590,128 @@ __device__ __forceinline__ void outside_site(const DevForest& F, const OutNode& 
         acc[i][2] = fma(A, f2, acc[i][2]);
         acc[i][4] = fma(sumG, f2, acc[i][4]);
         if (lf[i] && code[i] == -1)
-          rt = fma(n.phi * (av0 + fma(qs, Yv, Zv)), ldexp(iL, min(ea - LE, 1000)), rt);
+          rt = fma(phi * (av0 + fma(qs, Yv, Zv)), ldexp(iL, min(ea - LE, 1000)), rt);
       }
     }
+  }
+}
+
+// The posterior step a fused cherry's own node would run at site k, from the
+// outside state its parent just formed (au 2^Eo, gu): counts of its two leaf
+// branches into cacc, dropout responsibility of its missing leaves into crt.
+__device__ __forceinline__ void cherry_post(const DevForest& F, const ChildRows& r, int k, const double4& au, int Eo,
+                                            int gu, double iL, int LE, int qo, double qs, double phi,
+                                            double (&cacc)[2][5], double& crt) {
+  int lab[2], E[2] = {0, 0}, code[2];
+  double b0[2], bA[2], q[2];
+  Eps ep[2];
+  code[0] = row_at(r.code, k);
+  code[1] = row_at(r.code2, k);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    leaf_beta(code[j], phi, lab[j], b0[j], bA[j]);
+    q[j] = lab[j] > 0 ? row_at(F.prior, (unsigned)(qo + lab[j])) : qs;
+    ep[j] = edge(lab[j], b0[j], bA[j], 0, q[j], r.g[j]);
+  }
+  const bool lf[2] = {true, true};
+  post_site(au, Eo, gu, iL, LE, qs, phi, lab, b0, bA, E, code, q, ep, r.g, lf, cacc, crt,
+            [](int, const double4&, int, int) {});
+}
+
+// The outside step of one site in three phases: row loads, prior gathers,
+// then arithmetic and stores.
+struct OutSite {
+  double4 au;
+  int Eo, gu, LE, qo;
+  double Lm, qs;
+  int lab[2], E[2], code[2];
+  double b0[2], bA[2], q[2];
+};
+
+template <int K0, int K1>
+__device__ __forceinline__ void outside_load(const DevForest& F, const OutNode& n, int k, OutSite& x) {
+  const int kind[2] = {resolve_kind<K0>(n.kind[0]), resolve_kind<K1>(n.kind[1])};
+  x.au = row_at(n.a, k);
+  x.Eo = row_at(n.aE, k);
+  x.gu = row_at(n.aG, k);
+  x.Lm = row_at(n.Lm, k);
+  x.LE = row_at(n.LE, k);
+  x.qo = row_at(n.qoff, k) - 1;
+  x.qs = row_at(n.qsum, k);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+    load_child_rows(n.c[i], kind[i], k, n.phi, F.prior, x.qo, x.qs, x.lab[i], x.b0[i], x.bA[i], x.E[i], x.code[i]);
+}
+
+__device__ __forceinline__ void outside_gather(const DevForest& F, OutSite& x) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i) x.q[i] = x.lab[i] > 0 ? row_at(F.prior, (unsigned)(x.qo + x.lab[i])) : x.qs;
+}
+
+// One site k of node n: the children's outside states and the posterior
+// counts of both child branches, added to acc; the dropout responsibility of
+// missing leaf children added to rt.  A fused cherry child's outside state is
+// not stored: the cherry's own posterior step runs here, into cacc[i] and
+// crt[i].  K0 / K1: the kind of each child, or kNoCherry to read n.kind.
+template <int K0, int K1>
+__device__ __forceinline__ void outside_finish(const DevForest& F, const OutNode& n, int k, const OutSite& x,
+                                               double (&acc)[2][5], double& rt, double (&cacc)[2][2][5],
+                                               double (&crt)[2]) {
+  const int kind[2] = {resolve_kind<K0>(n.kind[0]), resolve_kind<K1>(n.kind[1])};
+  const bool lf[2] = {kind[0] == kLeaf, kind[1] == kLeaf};
+  const double iL = 1.0 / x.Lm;
+  Eps ep[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) ep[i] = edge(x.lab[i], x.b0[i], x.bA[i], x.E[i], x.q[i], n.cf[i]);
+  post_site(x.au, x.Eo, x.gu, iL, x.LE, x.qs, n.phi, x.lab, x.b0, x.bA, x.E, x.code, x.q, ep, n.cf, lf, acc, rt,
+            [&](int i, const double4& a, int ec, int gg) {
+              if (kind[i] == kCherry) {
+                cherry_post(F, n.c[i], k, a, ec, gg, iL, x.LE, x.qo, x.qs, n.phi, cacc[i], crt[i]);
+              } else {
+                row_at(n.ca[i], k) = a;
+                row_at(n.caE[i], k) = ec;
+                row_at(n.caG[i], k) = (int16_t)gg;
+              }
+            });
+}
+
+template <int K0, int K1>
+__device__ __forceinline__ void outside_site(const DevForest& F, const OutNode& n, int k, double (&acc)[2][5],
+                                             double& rt, double (&cacc)[2][2][5], double (&crt)[2]) {
+  OutSite x;
+  outside_load<K0, K1>(F, n, k, x);
+  outside_gather(F, x);
+  outside_finish<K0, K1>(F, n, k, x, acc, rt, cacc, crt);
+}
+
+// outside_site for kernels that never meet a fused cherry
+template <int K0, int K1>
+__device__ __forceinline__ void outside_site(const DevForest& F, const OutNode& n, int k, double (&acc)[2][5],
+                                             double& rt) {
+  double cacc[2][2][5];
+  double crt[2];
+  outside_site<K0, K1>(F, n, k, acc, rt, cacc, crt);
+}
+
+// segment sums of a fused cherry's leaf-branch counts and dropout sum for one
+// chunk; the last lane of the segment writes them as partial number `slot`
+// of the leaf branches and of the cherry.  All lanes of the segment call this.
+template <int W>
+__device__ __forceinline__ void cherry_write_partials(const DevForest& F, int c, const ChildRows& r, int lane,
+                                                      int slot, double (&cacc)[2][5], double crt) {
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int m = 0; m < 5; ++m) cacc[j][m] = seg_sum_last<W>(cacc[j][m]);
+  crt = seg_sum_last<W>(crt);
+  if (lane == W - 1) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      double* o = F.cpart + ((int64_t)r.gc[j] * F.max_chunks + slot) * 5;
+#pragma unroll
+      for (int m = 0; m < 5; ++m) o[m] = cacc[j][m];
+    }
+    F.rt_part[(int64_t)c * F.max_chunks + slot] = crt;
   }
 }
 
@@ -501,6 +741,13 @@ __device__ __forceinline__ void out_write_partials(const DevForest& F, int u, co
   }
 }
 
+__device__ __forceinline__ void zero_acc(double (&acc)[2][5]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[i][j] = 0.0;
+}
+
 // Segment form: the W lanes of a segment handle sites chunk*W + lane of node
 // u (all lanes of the segment call this together).
 template <int W>
@@ -513,43 +760,45 @@ __device__ __forceinline__ void outside_node(const DevForest& F, int u, int chun
   if (chunk * W >= K) return;
   if (done && done[p]) return;
   OutNode n;
-  out_node_load(F, u, t, p, K, n);
+  out_node_load<false>(F, u, t, p, K, n);
   double acc[2][5];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 5; ++j) acc[i][j] = 0.0;
+  zero_acc(acc);
   double rt = 0.0;
-  if (k < K) outside_site<-1, -1>(F, n, k, acc, rt);
+  if (k < K) outside_site<kNoCherry, kNoCherry>(F, n, k, acc, rt);
   out_write_partials<W>(F, u, n.ch, lane, chunk, chunk + 1, acc, rt);
 }
 
 // Row form (K > 32): one wavefront per node, so the node number is
-// wave-uniform (its metadata are scalar loads and the leaf tests scalar
-// branches); the wave walks chunks [c0, c0 + cpw) of 64 sites, accumulating
-// per lane, and reduces once.
-template <int LEAF0, int LEAF1>
+// wave-uniform (its metadata are scalar loads and the child-kind tests
+// scalar branches); the wave walks chunks [c0, c0 + cpw) of 64 sites,
+// accumulating per lane, and reduces once.  The leaf-branch counts of a
+// fused cherry child are reduced and written per chunk.
+template <int K0, int K1>
 __device__ __forceinline__ void outside_row_run(const DevForest& F, int u, const OutNode& n, int c0, int c1,
                                                 int lane) {
   double acc[2][5];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 5; ++j) acc[i][j] = 0.0;
+  zero_acc(acc);
   double rt = 0.0;
   for (int c = c0; c < c1; ++c) {
     const int k = c * 64 + lane;
-    if (k < n.K) outside_site<LEAF0, LEAF1>(F, n, k, acc, rt);
+    double cacc[2][2][5];
+    double crt[2] = {0.0, 0.0};
+    if (K0 == kCherry) zero_acc(cacc[0]);
+    if (K1 == kCherry) zero_acc(cacc[1]);
+    if (k < n.K) outside_site<K0, K1>(F, n, k, acc, rt, cacc, crt);
+    if (K0 == kCherry) cherry_write_partials<64>(F, n.ch[0], n.c[0], lane, c, cacc[0], crt[0]);
+    if (K1 == kCherry) cherry_write_partials<64>(F, n.ch[1], n.c[1], lane, c, cacc[1], crt[1]);
   }
   out_write_partials<64>(F, u, n.ch, lane, c0, c1, acc, rt);
 }
 
+// Nodes whose children are leaves or stored internal nodes.
 __global__ void __launch_bounds__(256) k_outside_rows(DevForest F, int beg, int cnt, int cpw,
-                                                       const int32_t* __restrict__ done) {
-  const int wv = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-  if (wv >= cnt) return;
+                                                       const int32_t* __restrict__ done,
+                                                       const int32_t* __restrict__ nodes) {
+  const int u = row_node(beg, cnt, nodes);
+  if (u < 0) return;
   const int lane = threadIdx.x & 63;
-  const int u = beg + wv;
   const int t = F.tree_of[u];
   const int p = F.tree_prob[t];
   const int K = F.tree_nsites[t];
@@ -559,13 +808,41 @@ __global__ void __launch_bounds__(256) k_outside_rows(DevForest F, int beg, int 
   if (done && done[p]) return;
   const int c1 = min(c0 + cpw, nch);
   OutNode n;
-  out_node_load(F, u, t, p, K, n);
-  if (n.lf[0]) {
-    if (n.lf[1]) outside_row_run<1, 1>(F, u, n, c0, c1, lane);
-    else outside_row_run<1, 0>(F, u, n, c0, c1, lane);
+  out_node_load<false>(F, u, t, p, K, n);
+  if (n.kind[0] == kLeaf) {
+    if (n.kind[1] == kLeaf) outside_row_run<kLeaf, kLeaf>(F, u, n, c0, c1, lane);
+    else outside_row_run<kLeaf, kInternal>(F, u, n, c0, c1, lane);
   } else {
-    if (n.lf[1]) outside_row_run<0, 1>(F, u, n, c0, c1, lane);
-    else outside_row_run<0, 0>(F, u, n, c0, c1, lane);
+    if (n.kind[1] == kLeaf) outside_row_run<kInternal, kLeaf>(F, u, n, c0, c1, lane);
+    else outside_row_run<kInternal, kInternal>(F, u, n, c0, c1, lane);
+  }
+}
+
+// Nodes with at least one fused cherry child.
+__global__ void __launch_bounds__(256) k_outside_rows_ch(DevForest F, int beg, int cnt, int cpw,
+                                                          const int32_t* __restrict__ done,
+                                                          const int32_t* __restrict__ nodes) {
+  const int u = row_node(beg, cnt, nodes);
+  if (u < 0) return;
+  const int lane = threadIdx.x & 63;
+  const int t = F.tree_of[u];
+  const int p = F.tree_prob[t];
+  const int K = F.tree_nsites[t];
+  const int nch = (K + 63) >> 6;
+  const int c0 = (int)blockIdx.y * cpw;
+  if (c0 >= nch) return;
+  if (done && done[p]) return;
+  const int c1 = min(c0 + cpw, nch);
+  OutNode n;
+  out_node_load<true>(F, u, t, p, K, n);
+  if (n.kind[0] == kCherry) {
+    if (n.kind[1] == kCherry) outside_row_run<kCherry, kCherry>(F, u, n, c0, c1, lane);
+    else if (n.kind[1] == kLeaf) outside_row_run<kCherry, kLeaf>(F, u, n, c0, c1, lane);
+    else outside_row_run<kCherry, kInternal>(F, u, n, c0, c1, lane);
+  } else if (n.kind[0] == kLeaf) {
+    outside_row_run<kLeaf, kCherry>(F, u, n, c0, c1, lane);
+  } else {
+    outside_row_run<kInternal, kCherry>(F, u, n, c0, c1, lane);
   }
 }
 
@@ -624,8 +901,8 @@ __global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWal
         const int v = W == 64 ? __builtin_amdgcn_readfirstlane(beg + i) : beg + i;
         if (k < K) {
           InNode n;
-          in_node_load(F, v, t, p, K, n);
-          inside_site<-1, -1>(F, n, k);
+          in_node_load<false>(F, v, t, p, K, n);
+          inside_site<kNoCherry, kNoCherry>(F, n, k);
         }
       }
       __syncthreads();
@@ -641,14 +918,14 @@ __global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWal
       for (int i = seg; i < cnt; i += kSegs) {
         const int u = W == 64 ? __builtin_amdgcn_readfirstlane(beg + i) : beg + i;
         OutNode n;
-        out_node_load(F, u, t, p, K, n);
+        out_node_load<false>(F, u, t, p, K, n);
         double acc[2][5];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int j = 0; j < 5; ++j) acc[a][j] = 0.0;
         double rt = 0.0;
-        if (k < K) outside_site<-1, -1>(F, n, k, acc, rt);
+        if (k < K) outside_site<kNoCherry, kNoCherry>(F, n, k, acc, rt);
         out_write_partials<W>(F, u, n.ch, lane, chunk, chunk + 1, acc, rt);
       }
       __syncthreads();
@@ -1568,6 +1845,16 @@ void run_coef(const DevForest& F, const double* d, const double* nu, const int32
   hipLaunchKernelGGL(k_coef, dim3(grid), dim3(256), 0, s, F, d, nu, done);
 }
 
+static void launch_inside_rows(dim3 grid, hipStream_t s, const DevForest& F, int beg, int cnt, int cpw,
+                               const int32_t* done, const int32_t* nodes) {
+  hipLaunchKernelGGL(k_inside_rows, grid, dim3(256), 0, s, F, beg, cnt, cpw, done, nodes);
+}
+
+static void launch_outside_rows(dim3 grid, hipStream_t s, const DevForest& F, int beg, int cnt, int cpw,
+                                const int32_t* done, const int32_t* nodes) {
+  hipLaunchKernelGGL(k_outside_rows, grid, dim3(256), 0, s, F, beg, cnt, cpw, done, nodes);
+}
+
 // one level of the inside pass: row kernels when a per-level shape is given
 // (W == 64), segment kernels otherwise
 static void inside_level(const DevForest& F, const std::vector<int32_t>& lvl, int W, const std::vector<int32_t>& cpw,
@@ -1576,8 +1863,7 @@ static void inside_level(const DevForest& F, const std::vector<int32_t>& lvl, in
   if (cnt <= 0) return;
   if (W == 64 && !cpw.empty() && cpw[h] > 0) {
     const int c = cpw[h];
-    hipLaunchKernelGGL(k_inside_rows, dim3((cnt + 3) / 4, (F.max_chunks + c - 1) / c), dim3(256), 0, s, F, beg, cnt,
-                       c, done);
+    launch_inside_rows(dim3((cnt + 3) / 4, (F.max_chunks + c - 1) / c), s, F, beg, cnt, c, done, nullptr);
     return;
   }
   LAML_DISPATCH_W(W, {
@@ -1591,8 +1877,7 @@ static void outside_level(const DevForest& F, const std::vector<int32_t>& lvl, i
   if (cnt <= 0) return;
   if (W == 64 && !cpw.empty() && cpw[h] > 0) {
     const int c = cpw[h];
-    hipLaunchKernelGGL(k_outside_rows, dim3((cnt + 3) / 4, (F.max_chunks + c - 1) / c), dim3(256), 0, s, F, beg,
-                       cnt, c, done);
+    launch_outside_rows(dim3((cnt + 3) / 4, (F.max_chunks + c - 1) / c), s, F, beg, cnt, c, done, nullptr);
     return;
   }
   LAML_DISPATCH_W(W, {
@@ -1615,9 +1900,38 @@ std::vector<int32_t> default_cpw(const std::vector<int32_t>& lvl, int W, int max
   return c;
 }
 
+// One level of a pass with fused cherries (row kernels): the level's nodes
+// without a cherry child, then those with one, each from its node list.
+static void split_level(bool outside, const DevForest& F, const CherrySplit& cs, const std::vector<int32_t>& cpw,
+                        int h, const int32_t* done, hipStream_t s) {
+  const int c = std::max(1, cpw[h]);
+  const int pb = cs.off[2 * h], pe = cs.off[2 * h + 1], ce = cs.off[2 * h + 2];
+  const unsigned gy = (unsigned)((F.max_chunks + c - 1) / c);
+  if (pe > pb) {
+    const int cnt = pe - pb;
+    if (outside)
+      launch_outside_rows(dim3((cnt + 3) / 4, gy), s, F, pb, cnt, c, done, cs.nodes);
+    else
+      launch_inside_rows(dim3((cnt + 3) / 4, gy), s, F, pb, cnt, c, done, cs.nodes);
+  }
+  if (ce > pe) {
+    const int cnt = ce - pe;
+    if (outside)
+      hipLaunchKernelGGL(k_outside_rows_ch, dim3((cnt + 3) / 4, gy), dim3(256), 0, s, F, pe, cnt, c, done,
+                         cs.nodes);
+    else
+      hipLaunchKernelGGL(k_inside_rows_ch, dim3((cnt + 3) / 4, gy), dim3(256), 0, s, F, pe, cnt, c, done, cs.nodes);
+  }
+}
+
 void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const std::vector<int32_t>& cpw,
-                const int32_t* done, hipStream_t s) {
+                const int32_t* done, hipStream_t s, const CherrySplit* cs) {
   const int H = (int)lvl.size() - 1;
+  if (cs) {
+    // fused cherries: row kernels on every level from 2 up, no top kernel
+    for (int h = 2; h < H; ++h) split_level(false, F, *cs, cpw, h, done, s);
+    return;
+  }
   const int ht = std::max(1, std::min(h_top, H));
   for (int h = 1; h < ht; ++h) inside_level(F, lvl, W, cpw, h, done, s);
   if (ht < H) {
@@ -1650,8 +1964,12 @@ void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W
 }
 
 void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top,
-                 const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s) {
+                 const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s, const CherrySplit* cs) {
   const int H = (int)lvl.size() - 1;
+  if (cs) {
+    for (int h = H - 1; h >= 2; --h) split_level(true, F, *cs, cpw, h, done, s);
+    return;
+  }
   const int ht = std::max(1, std::min(h_top, H));
   if (ht < H) {
     LAML_DISPATCH_W(W, {

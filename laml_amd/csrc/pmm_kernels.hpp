@@ -21,6 +21,10 @@ struct DevForest {
   int32_t num_nodes, num_leaves, num_trees, num_problems;
   int32_t chunk_w;     // sites per segment (W)
   int32_t max_chunks;  // ceil(max K / W)
+  // nodes [num_leaves, cherry_end) are fused cherries in the current pass
+  // (their parents form and consume their inside and outside states; no
+  // cherry row is read or written); cherry_end == num_leaves: no fusion
+  int32_t cherry_end;
   double dmin, dmax;
   // topology (level order)
   const int32_t* level_off;  // [levels+1]
@@ -132,10 +136,18 @@ constexpr int64_t kRowTargetWaves = 8192;
 // default chunks per wavefront of the row kernels for each level (empty
 // unless W == 64)
 std::vector<int32_t> default_cpw(const std::vector<int32_t>& lvl, int W, int max_chunks);
+// Per-level node lists of a pass with fused cherries: for level h >= 2,
+// nodes[off[2h] .. off[2h+1]) have no cherry child, nodes[off[2h+1] ..
+// off[2h+2]) have at least one (device list, host offsets).
+struct CherrySplit {
+  const int32_t* nodes = nullptr;
+  std::vector<int32_t> off;
+};
+
 void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const std::vector<int32_t>& cpw,
-                const int32_t* done, hipStream_t s);
+                const int32_t* done, hipStream_t s, const CherrySplit* cs = nullptr);
 void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top,
-                 const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s);
+                 const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s, const CherrySplit* cs = nullptr);
 void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W, int L,
                       const std::vector<int32_t>& in_cpw, const std::vector<int32_t>& out_cpw, const int32_t* done,
                       hipStream_t s);

@@ -513,6 +513,17 @@ class Forest(ForestLayout):
     def pass_mode(self):
         return int(self._lib.laml_forest_pass_mode(self.handle))
 
+    def set_fusion(self, on):
+        """Cherry fusion of the level passes (off by default): the nodes
+        whose children are both leaves are formed and consumed by their
+        parents.  posteriors() and quartet_em() need an E-step run with it
+        off."""
+        _lib.check(self._lib.laml_forest_set_fusion(self.handle, 1 if on else 0))
+
+    def fusion(self):
+        """Whether the next level pass fuses the cherries."""
+        return bool(self._lib.laml_forest_fusion(self.handle))
+
     def get_shapes(self):
         """Launch shapes: (inside_cpw, outside_cpw, top_level); the cpw arrays
         hold, per level, the 64-site chunks one wavefront walks (all 0 when

@@ -98,6 +98,8 @@ class QuartetScorer:
         d, nu, phi = s0._device_params(F)
         nu.fill_(self.nu)
         phi.fill_(self.phi)
+        # quartet EM reads every node's rows: no cherry fusion in this E-step
+        F.set_fusion(False)
         F.nll(d, nu, phi, site_llh=True)
         sl = F.site_llh.cpu().numpy()
         self.tree_llh = [float(sl[F.site_base[i]:F.site_base[i + 1]].sum()) for i in range(F.num_trees)]

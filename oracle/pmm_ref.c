@@ -372,20 +372,29 @@ static int is_root_count(const pmm_problem* P) {
 int pmm_ref_threads(void) { return omp_get_max_threads(); }
 
 /* lineage_llh: -sum over trees and sites of root.L0 */
+/* per-thread partial sums are combined in thread order, so results do not
+ * depend on which thread finishes first */
 double pmm_ref_nll(const pmm_problem* P, const double* d, double nu, double phi) {
-  double llh = 0.0;
-#pragma omp parallel reduction(+ : llh)
+  const int T = omp_get_max_threads();
+  double* part = (double*)calloc(T, sizeof(double));
+#pragma omp parallel
   {
+    const int t = omp_get_thread_num();
     site_ws w;
     ws_init(&w, P->n);
+    double llh = 0.0;
 #pragma omp for schedule(static)
     for (int k = 0; k < P->K; ++k) {
       inside_site(P, d, nu, phi, k, &w);
       for (int v = 0; v < P->n; ++v)
         if (P->parent[v] < 0) llh += w.L0[v];
     }
+    part[t] = llh;
     ws_free(&w);
   }
+  double llh = 0.0;
+  for (int t = 0; t < T; ++t) llh += part[t];
+  free(part);
   return -llh;
 }
 
@@ -398,24 +407,29 @@ double pmm_ref_estep(const pmm_problem* P, const double* d, double nu, double ph
   const int T = omp_get_max_threads();
   double* acc = (double*)calloc((size_t)T * n * 5, sizeof(double));
   double* rt_acc = (double*)calloc(T, sizeof(double));
-  double llh = 0.0;
-#pragma omp parallel reduction(+ : llh)
+  double* llh_acc = (double*)calloc(T, sizeof(double));
+#pragma omp parallel
   {
     const int t = omp_get_thread_num();
     site_ws w;
     ws_init(&w, n);
     int* path = (int*)malloc(sizeof(int) * n);
+    double llh_t = 0.0;
 #pragma omp for schedule(static)
     for (int k = 0; k < P->K; ++k) {
       inside_site(P, d, nu, phi, k, &w);
       for (int v = 0; v < n; ++v)
-        if (P->parent[v] < 0) llh += w.L0[v];
+        if (P->parent[v] < 0) llh_t += w.L0[v];
       outside_site(P, d, nu, k, &w, path);
       posterior_site(P, d, nu, phi, k, &w, acc + (size_t)t * n * 5, &rt_acc[t]);
     }
+    llh_acc[t] = llh_t;
     free(path);
     ws_free(&w);
   }
+  double llh = 0.0;
+  for (int t = 0; t < T; ++t) llh += llh_acc[t];
+  free(llh_acc);
   memset(counts, 0, sizeof(double) * 5 * (size_t)n);
   double rt = 0.0;
   for (int t = 0; t < T; ++t) {
