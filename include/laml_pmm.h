@@ -105,7 +105,8 @@ int laml_forest_set_flags(laml_forest* f, const uint32_t* flags_host);
  * (and the roots) are kept; the outside pass walks the blocks top-down and
  * recomputes each block's inside vectors in one reusable working area before
  * using them.  levels = 0 restores full storage.  Inside state must be
- * recomputed (laml_nll) after a change.
+ * recomputed (laml_nll) after a change.  Needs the level pass mode (a forest
+ * in the hybrid mode is switched to it; the tree-walk mode is refused).
  * No counterpart in the reference (its inside values live in Python
  * objects); it bounds the memory of the adjoint for very large trees. */
 int laml_forest_set_checkpoint(laml_forest* f, int32_t levels);
@@ -123,16 +124,24 @@ int64_t laml_forest_inside_rows(laml_forest* f);
  * floating-point sums over sites only.
  * No counterpart in the reference (its passes are Python loops); the values
  * come from the default heuristic or from laml_amd.autotune. */
-/* Pass mode: LAML_PASS_LEVELS (default) runs each pass as level-synchronous
- * launches over the whole forest; LAML_PASS_TREES runs the inside pass, the
- * root step and the outside pass of every (tree, chunk of sites) pair in one
- * block of a single launch, which suits forests of many small or uneven
- * trees.  The tree walk needs full inside storage (no level checkpointing).
- * Both modes compute the same values; only the order of floating-point sums
- * over sites may differ.
+/* Pass mode: LAML_PASS_LEVELS runs each pass as level-synchronous launches
+ * over the whole forest; LAML_PASS_TREES runs the inside pass, the root step
+ * and the outside pass of every (tree, chunk of sites) pair in one block of a
+ * single launch, which suits forests of many small or uneven trees;
+ * LAML_PASS_HYBRID (the default; environment LAML_PASS_MODE=0 or 1 selects
+ * another default) combines them.  The tree walk needs full inside storage:
+ * LAML_PASS_TREES refuses level checkpointing, and turning checkpointing on in
+ * LAML_PASS_HYBRID switches the forest to LAML_PASS_LEVELS.  All modes
+ * compute the same values; only the order of floating-point sums over sites
+ * may differ.
  * No counterpart in the reference. */
 #define LAML_PASS_LEVELS 0
 #define LAML_PASS_TREES 1
+/* LAML_PASS_HYBRID: level launches for the levels below top_level (see
+ * laml_forest_set_shapes) and one tree-walk launch for the levels from
+ * top_level up, the root step included: the small upper levels of every tree
+ * run in one launch instead of one launch each. */
+#define LAML_PASS_HYBRID 2
 int laml_forest_set_pass_mode(laml_forest* f, int32_t mode);
 int32_t laml_forest_pass_mode(laml_forest* f);
 

@@ -14,6 +14,7 @@ LAML_OPT_PHI = 1
 LAML_OPT_NU = 2
 PASS_LEVELS = 0
 PASS_TREES = 1
+PASS_HYBRID = 2
 
 CODE_UNEDITED = 0
 CODE_MISSING = -1

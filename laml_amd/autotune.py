@@ -3,21 +3,22 @@
 A forest's passes can run as level-synchronous launches (per level: the
 number of 64-site chunks one wavefront walks, whether the small top levels
 share one fused launch, and whether the cherries -- nodes with two leaf
-children -- are fused into their parents) or as the tree walk (one launch per
-pass, a block per (tree, site chunk)).  Which is fastest depends on the forest: how
+children -- are fused into their parents), as the tree walk (one launch per
+pass, a block per (tree, site chunk)), or as a hybrid of the two (level
+launches for the big bottom levels, one walk for the small top ones).  Which is fastest depends on the forest: how
 many trees, how deep, how many sites, how uneven.  ``autotune`` times the
 E-step (inside pass, root, outside pass, counts) under candidate plans on
 the forest itself and keeps the fastest; every plan computes the same values
 up to the order of floating-point sums over sites.
 
-The search is a coarse pass over pass mode, top-level fusion and cherry
-fusion, then a coordinate search over the chunks per wavefront of the largest
-levels.
+The search is a coarse pass over pass mode (with a few hybrid splits),
+top-level fusion and cherry fusion, then a coordinate search over the chunks
+per wavefront of the largest levels.
 """
 import numpy as np
 import torch
 
-from ._lib import PASS_LEVELS, PASS_TREES
+from ._lib import PASS_HYBRID, PASS_LEVELS, PASS_TREES
 
 
 def _time_estep(F, d, nu, phi, reps):
@@ -57,8 +58,9 @@ class Plan:
     def describe(self):
         if self.mode == PASS_TREES:
             return {"mode": "trees"}
-        return {"mode": "levels", "top_level": self.top_level, "cherry_fusion": self.fuse,
-                "inside_cpw": self.inside_cpw.tolist(), "outside_cpw": self.outside_cpw.tolist()}
+        return {"mode": "hybrid" if self.mode == PASS_HYBRID else "levels", "top_level": self.top_level,
+                "cherry_fusion": self.fuse, "inside_cpw": self.inside_cpw.tolist(),
+                "outside_cpw": self.outside_cpw.tolist()}
 
 
 def current_plan(F):
@@ -87,10 +89,11 @@ def autotune(F, d, nu, phi, reps=3, max_levels=8, verbose=False):
 
     best = base
     best_ms = measure(base, "default")
-    cands = [("no fused top", Plan(PASS_LEVELS, base.inside_cpw, base.outside_cpw, H, base.fuse)),
-             ("tree walk", Plan(PASS_TREES, base.inside_cpw, base.outside_cpw, base.top_level, base.fuse))]
-    # cherry fusion, where the forest allows it (it replaces the fused top
-    # kernels: the top levels run as row launches)
+    top0 = base.top_level
+    cands = [("levels, fused top", Plan(PASS_LEVELS, base.inside_cpw, base.outside_cpw, top0, False)),
+             ("levels", Plan(PASS_LEVELS, base.inside_cpw, base.outside_cpw, H, False)),
+             ("tree walk", Plan(PASS_TREES, base.inside_cpw, base.outside_cpw, top0, False))]
+    # cherry fusion, where the forest allows it (level mode, no fused top)
     mode = F.pass_mode()
     F.set_pass_mode(PASS_LEVELS)
     F.set_fusion(True)
@@ -98,14 +101,27 @@ def autotune(F, d, nu, phi, reps=3, max_levels=8, verbose=False):
     F.set_fusion(base.fuse)
     F.set_pass_mode(mode)
     if can_fuse:
-        cands.append(("cherry fusion" if not base.fuse else "no cherry fusion",
-                      Plan(PASS_LEVELS, base.inside_cpw, base.outside_cpw, H, not base.fuse)))
+        cands.append(("cherry fusion", Plan(PASS_LEVELS, base.inside_cpw, base.outside_cpw, H, True)))
+    # hybrid: the levels from a split up in one tree-walk launch; splits where
+    # the levels get small
+    sizes = np.diff(F.level_off)
+    splits = set()
+    for cap in (64, 256, 1024, 4096):
+        h = H
+        while h - 1 >= 2 and sizes[h - 1] <= cap:
+            h -= 1
+        if 2 <= h < H:
+            splits.add(int(h))
+    for h in sorted(splits):
+        if base.mode == PASS_HYBRID and h == base.top_level:
+            continue
+        cands.append(("hybrid from level %d" % h, Plan(PASS_HYBRID, base.inside_cpw, base.outside_cpw, h, False)))
     for tag, plan in cands:
         ms = measure(plan, tag)
         if ms < best_ms:
             best, best_ms = plan, ms
     rows = base.inside_cpw[1:].max() > 0  # row kernels in use (more than 32 sites)
-    if best.mode == PASS_LEVELS and rows:
+    if best.mode in (PASS_LEVELS, PASS_HYBRID) and rows:
         sizes = np.diff(F.level_off)
         first = 2 if best.fuse else 1  # the cherry level has no launch of its own when fused
         big = [h for h in np.argsort(-sizes) if first <= h < min(H, best.top_level)][:max_levels]

@@ -69,8 +69,9 @@ struct laml_forest {
   int ckpt_L = 0;       // level-checkpoint segment length (0: all inside vectors stored)
   // per level: 64-site chunks one wavefront of the row kernels walks (W == 64)
   std::vector<int32_t> in_cpw, out_cpw;
-  // pass mode: 0 = level-synchronous launches, 1 = tree walk (one launch)
-  int pass_mode = 0;
+  // pass mode: 0 = level-synchronous launches, 1 = tree walk (one launch),
+  // 2 = hybrid (level launches below h_top, one tree walk above; default)
+  int pass_mode = 2;
   // cherry fusion (the height-1 level formed and consumed by its parents):
   // requested (off by default; the autotuner tries it), possible for this
   // forest (row kernels, every tree with at least three leaves), and
@@ -394,6 +395,8 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
     while (ht - 1 >= 1 && d->level_off[ht] - d->level_off[ht - 1] <= top_max) --ht;
     f->h_top = ht;
     f->h_top_default = ht;
+    const char* pm = getenv("LAML_PASS_MODE");
+    if (pm && (atoi(pm) == 0 || atoi(pm) == 1)) f->pass_mode = atoi(pm);
   }
   f->leaf_rows = leaf_rows;
   f->int_rows = int_rows;
@@ -675,6 +678,9 @@ int laml_forest_set_checkpoint(laml_forest* f, int32_t levels) {
   if (levels < 0) return fail(LAML_EINVAL, "negative checkpoint length");
   if (levels > 0 && f->pass_mode == LAML_PASS_TREES)
     return fail(LAML_ESTATE, "level checkpointing needs the level pass mode");
+  // the checkpointed outside pass walks level blocks: the hybrid mode's walk
+  // over the top levels gives way to level launches
+  if (levels > 0 && f->pass_mode == LAML_PASS_HYBRID) f->pass_mode = LAML_PASS_LEVELS;
   DeviceGuard guard(f->device);
   LAML_HIP(hipDeviceSynchronize());
   const int N = f->F.num_nodes, n = f->F.num_leaves;
@@ -759,8 +765,9 @@ int laml_forest_get_shapes(laml_forest* f, int32_t* inside_cpw, int32_t* outside
 
 int laml_forest_set_pass_mode(laml_forest* f, int32_t mode) {
   if (!f) return fail(LAML_EINVAL, "null forest");
-  if (mode != LAML_PASS_LEVELS && mode != LAML_PASS_TREES) return fail(LAML_EINVAL, "unknown pass mode");
-  if (mode == LAML_PASS_TREES && f->ckpt_L > 0)
+  if (mode != LAML_PASS_LEVELS && mode != LAML_PASS_TREES && mode != LAML_PASS_HYBRID)
+    return fail(LAML_EINVAL, "unknown pass mode");
+  if (mode != LAML_PASS_LEVELS && f->ckpt_L > 0)
     return fail(LAML_ESTATE, "the tree walk needs full inside storage (level checkpointing is on)");
   f->pass_mode = mode;
   f->plan_version++;
@@ -909,10 +916,22 @@ static DevForest with_cherries(const laml_forest* f, const DevForest& F, bool fu
   return G;
 }
 
+// hybrid pass mode: level launches below this level, one tree-walk launch
+// (a block per tree and site chunk) for the levels from it up
+static int hybrid_split(const laml_forest* f) {
+  const int H = (int)f->level_off.size() - 1;
+  return std::max(1, std::min(f->h_top, H));
+}
+
 // inside pass and root step (site likelihoods) in the forest's pass mode
 static void pass_inside_root(laml_forest* f, const DevForest& F, double* site_llh, const int32_t* done,
                              hipStream_t s) {
-  if (f->pass_mode == 1) {
+  if (f->pass_mode == LAML_PASS_HYBRID) {
+    f->inside_fused = false;
+    const int hs = hybrid_split(f);
+    laml::run_inside_levels(F, f->level_off, f->W, 1, hs, f->in_cpw, done, s);
+    laml::run_tree_walk(F, f->TW, f->W, 3, site_llh, done, s, hs);
+  } else if (f->pass_mode == 1) {
     f->inside_fused = false;
     laml::run_tree_walk(F, f->TW, f->W, 3, site_llh, done, s);
   } else {
@@ -935,6 +954,12 @@ static void pass_outside(laml_forest* f, const DevForest& F, bool with_root, con
   }
   if (f->pass_mode == 1) {
     laml::run_tree_walk(F, f->TW, f->W, (with_root ? 2 : 0) | 4, nullptr, done, s);
+    return;
+  }
+  if (f->pass_mode == LAML_PASS_HYBRID) {
+    const int hs = hybrid_split(f);
+    laml::run_tree_walk(F, f->TW, f->W, (with_root ? 2 : 0) | 4, nullptr, done, s, hs);
+    laml::run_outside_levels(F, f->level_off, f->W, 1, hs, f->out_cpw, done, s);
     return;
   }
   if (with_root) laml::run_root(F, f->W, nullptr, done, s);
@@ -969,6 +994,12 @@ int laml_estep(laml_forest* f, const double* d, const double* nu, const double* 
   if (f->pass_mode == 1) {
     f->inside_fused = false;
     laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, nullptr, s);
+  } else if (f->pass_mode == LAML_PASS_HYBRID) {
+    f->inside_fused = false;
+    const int hs = hybrid_split(f);
+    laml::run_inside_levels(F, f->level_off, f->W, 1, hs, f->in_cpw, nullptr, s);
+    laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, nullptr, s, hs);
+    laml::run_outside_levels(F, f->level_off, f->W, 1, hs, f->out_cpw, nullptr, s);
   } else {
     pass_inside_root(f, F, nullptr, nullptr, s);
     pass_outside(f, F, false, nullptr, s);
@@ -1025,6 +1056,15 @@ static void em_step(laml_forest* f, const DevForest& F, const EmState& em, doubl
     f->inside_fused = false;
     laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, em.done, s);
     laml::run_llh(F, nll, em, max_iter, tol, s);
+  } else if (f->pass_mode == LAML_PASS_HYBRID && !last) {
+    // bottom inside levels, one walk for the top of both passes, the
+    // convergence test, then the bottom outside levels
+    f->inside_fused = false;
+    const int hs = hybrid_split(f);
+    laml::run_inside_levels(F, f->level_off, f->W, 1, hs, f->in_cpw, em.done, s);
+    laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, em.done, s, hs);
+    laml::run_llh(F, nll, em, max_iter, tol, s);
+    laml::run_outside_levels(F, f->level_off, f->W, 1, hs, f->out_cpw, em.done, s);
   } else {
     pass_inside_root(f, F, nullptr, em.done, s);
     laml::run_llh(F, nll, em, max_iter, tol, s);

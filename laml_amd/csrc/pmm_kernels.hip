@@ -878,9 +878,10 @@ constexpr int kWalkThreads = 1024;
 // never depend on other chunks, so blocks are independent: a forest of many
 // trees of uneven sizes is one launch per pass, and each block stops at its
 // own tree's height.  phases: 1 inside, 2 root, 4 outside (any combination,
-// in that order).
+// in that order).  Levels below h_lo are left to level launches (the hybrid
+// pass: the walk covers the small top levels of every tree at once).
 template <int W>
-__global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWalk TW, int phases,
+__global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWalk TW, int phases, int h_lo,
                                                             double* __restrict__ site_llh_out,
                                                             const int32_t* __restrict__ done) {
   const int t = TW.blk_tree[blockIdx.x];
@@ -894,8 +895,9 @@ __global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWal
   const int k = chunk * W + lane;
   const int lb = TW.lvl_base[t];
   const int H = TW.lvl_base[t + 1] - lb;
+  const int hb = max(1, h_lo);
   if (phases & 1) {
-    for (int h = 1; h < H; ++h) {
+    for (int h = hb; h < H; ++h) {
       const int beg = TW.lvl_beg[lb + h], cnt = TW.lvl_cnt[lb + h];
       for (int i = seg; i < cnt; i += kSegs) {
         const int v = W == 64 ? __builtin_amdgcn_readfirstlane(beg + i) : beg + i;
@@ -913,7 +915,7 @@ __global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWal
     __syncthreads();
   }
   if (phases & 4) {
-    for (int h = H - 1; h >= 1; --h) {
+    for (int h = H - 1; h >= hb; --h) {
       const int beg = TW.lvl_beg[lb + h], cnt = TW.lvl_cnt[lb + h];
       for (int i = seg; i < cnt; i += kSegs) {
         const int u = W == 64 ? __builtin_amdgcn_readfirstlane(beg + i) : beg + i;
@@ -1980,11 +1982,21 @@ void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int
 }
 
 void run_tree_walk(const DevForest& F, const TreeWalk& TW, int W, int phases, double* site_llh_out,
-                   const int32_t* done, hipStream_t s) {
+                   const int32_t* done, hipStream_t s, int h_lo) {
   LAML_DISPATCH_W(W, {
-    hipLaunchKernelGGL(k_tree_walk<WW>, dim3(TW.num_blocks), dim3(kWalkThreads), 0, s, F, TW, phases, site_llh_out,
-                       done);
+    hipLaunchKernelGGL(k_tree_walk<WW>, dim3(TW.num_blocks), dim3(kWalkThreads), 0, s, F, TW, phases, h_lo,
+                       site_llh_out, done);
   });
+}
+
+void run_inside_levels(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h0, int h1,
+                       const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s) {
+  for (int h = h0; h < h1; ++h) inside_level(F, lvl, W, cpw, h, done, s);
+}
+
+void run_outside_levels(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h0, int h1,
+                        const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s) {
+  for (int h = h1 - 1; h >= h0; --h) outside_level(F, lvl, W, cpw, h, done, s);
 }
 
 // Posterior state probabilities of every (node, site): P(v = 0 | D) and

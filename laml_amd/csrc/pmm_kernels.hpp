@@ -154,7 +154,12 @@ void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W
 void run_root(const DevForest& F, int W, double* site_llh_out, const int32_t* done, hipStream_t s);
 void run_counts(const DevForest& F, double* counts, hipStream_t s);
 void run_tree_walk(const DevForest& F, const TreeWalk& TW, int W, int phases, double* site_llh_out,
-                   const int32_t* done, hipStream_t s);
+                   const int32_t* done, hipStream_t s, int h_lo = 1);
+// levels [h0, h1) of the inside pass (bottom up) / outside pass (top down)
+void run_inside_levels(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h0, int h1,
+                       const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s);
+void run_outside_levels(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h0, int h1,
+                        const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s);
 void run_llh(const DevForest& F, double* nll, EmState em, int max_iter, double tol, hipStream_t s);
 void run_em_reset(const EmState& em, int P, hipStream_t s);
 void run_node_post(const DevForest& F, int W, int64_t leaf_rows, double* p0, double* pS, hipStream_t s);

@@ -506,8 +506,9 @@ class Forest(ForestLayout):
         return lo, nl, it
 
     def set_pass_mode(self, mode):
-        """0 (levels) or 1 (tree walk: one launch per pass, a block per
-        (tree, site chunk))."""
+        """0 (levels), 1 (tree walk: one launch per pass, a block per (tree,
+        site chunk)) or 2 (hybrid: level launches below the top level of the
+        launch shapes, one tree-walk launch above)."""
         _lib.check(self._lib.laml_forest_set_pass_mode(self.handle, int(mode)))
 
     def pass_mode(self):

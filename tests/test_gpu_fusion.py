@@ -25,6 +25,9 @@ def _pair(problems):
     dev = torch.cuda.current_device()
     Fa = Forest(specs, device=dev)
     Fb = Forest(specs, device=dev)
+    # fusion runs in the level pass mode
+    Fa.set_pass_mode(0)
+    Fb.set_pass_mode(0)
     assert not Fb.fusion()
     Fa.set_fusion(True)
     rows = max(len(q) for (_, q, _, _) in problems) > 32
@@ -149,6 +152,7 @@ def test_fusion_eligibility_and_refusals():
     # a two-leaf tree has its root on the cherry level: no fusion
     two = sim.problem_spec(*sim.make_problem(2, 30, 3, 10, 1.0, 0.1, 0.1, 991)[:3])
     F2 = Forest([two], device=torch.cuda.current_device())
+    F2.set_pass_mode(0)
     F2.set_fusion(True)
     assert not F2.fusion()
     F2.close()

@@ -84,8 +84,49 @@ def test_autotuned_plan_gives_the_same_results():
     F, d, nu, phi, _ = _forest([800, 50, 2000, 300], 140, 31)
     counts0, rt0, nll0 = [x.cpu().numpy().copy() for x in F.estep(d, nu, phi)]
     plan, log = autotune(F, d, nu, phi, reps=2)
-    assert len(log) >= 3 and plan.describe()["mode"] in ("levels", "trees")
+    assert len(log) >= 3 and plan.describe()["mode"] in ("levels", "trees", "hybrid")
     counts1, rt1, nll1 = [x.cpu().numpy().copy() for x in F.estep(d, nu, phi)]
     np.testing.assert_allclose(nll1, nll0, rtol=1e-12)
     np.testing.assert_allclose(counts1, counts0, rtol=1e-10, atol=1e-11)
     np.testing.assert_allclose(rt1, rt0, rtol=1e-10, atol=1e-11)
+
+
+@pytest.mark.parametrize("K", [13, 64, 130])
+def test_hybrid_equals_levels(K):
+    """The hybrid mode (level launches below a split level, one tree walk for
+    the levels above it, root included) at every split gives the level
+    mode's likelihood, counts, gradient and EM trajectory."""
+    F, d, nu, phi, o_nll = _forest([2, 3, 40, 257, 900, 5, 120], K, 700 + K)
+    H = F.num_levels
+    a0, b0, top0 = F.get_shapes()
+    F.set_pass_mode(0)
+    F.set_shapes(a0, b0, H)
+    ref = (F.nll(d, nu, phi).cpu().numpy().copy(),) + tuple(x.cpu().numpy().copy() for x in F.estep(d, nu, phi)) \
+        + tuple(x.cpu().numpy().copy() for x in F.grad(d, nu, phi))
+    for split in (1, 2, 3, H // 2, H - 1, H):
+        F.set_pass_mode(0)
+        F.set_shapes(a0, b0, split)
+        F.set_pass_mode(2)
+        print("hybrid split", split, "of", H, flush=True)
+        got = (F.nll(d, nu, phi).cpu().numpy().copy(),) + tuple(x.cpu().numpy().copy() for x in F.estep(d, nu, phi)) \
+            + tuple(x.cpu().numpy().copy() for x in F.grad(d, nu, phi))
+        for a, b in zip(ref, got):
+            np.testing.assert_allclose(b, a, rtol=1e-11, atol=1e-11)
+    for got, want in zip(ref[0], o_nll):
+        if want is not None:
+            assert math.isclose(got, want, rel_tol=1e-9)
+    # the EM loop (captured step) in hybrid mode follows the level mode
+    res = {}
+    for mode, split in ((0, H), (2, max(2, H // 2))):
+        F.set_pass_mode(0)
+        F.set_shapes(a0, b0, split)
+        F.set_pass_mode(mode)
+        dd, nn, pp = d.clone(), nu.clone(), phi.clone()
+        nll, it = F.em(dd, nn, pp, 15, 0.0)
+        res[mode] = (nll.cpu().numpy().copy(), dd.cpu().numpy().copy(), nn.cpu().numpy().copy())
+    np.testing.assert_allclose(res[2][0], res[0][0], rtol=1e-11)
+    np.testing.assert_allclose(res[2][1], res[0][1], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(res[2][2], res[0][2], rtol=1e-9)
+    F.set_pass_mode(0)
+    with pytest.raises(Exception):
+        F.set_pass_mode(3)
