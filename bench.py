@@ -603,7 +603,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": PROFILE if traffic is not None else None,
-                     "kernel": "outside pass (k_outside levels + k_root + k_rtilde)",
+                     "kernel": "outside pass (laml_outside: k_tree_walk top levels + root in the hybrid plan, k_outside_rows levels, k_counts, k_rtilde)",
                      "algorithmic_bytes": outside_b, "ms": round(out_ms, 4),
                      "inside_pass": {"algorithmic_bytes": inside_b, "ms": round(in_ms, 4)}},
     }
