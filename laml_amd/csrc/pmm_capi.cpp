@@ -414,6 +414,13 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   F.max_chunks = (max_k + f->W - 1) / f->W;
   F.cherry_end = d->num_leaves;
   {
+    // stream the rows past the caches when one pass's inside + outside rows
+    // (60 B per internal site) exceed the 256 MiB last-level cache
+    const char* env = getenv("LAML_STREAM_ROWS");
+    const double row_bytes = 60.0 * (double)int_rows;
+    F.stream_rows = env ? (atoi(env) != 0) : (row_bytes > 256.0 * 1024 * 1024);
+  }
+  {
     // the height-1 level holds exactly the nodes whose children are both
     // leaves; fusing it needs every tree root above it and row kernels
     bool ok = d->num_levels >= 3 && f->W == 64;

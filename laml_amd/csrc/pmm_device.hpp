@@ -114,6 +114,25 @@ __device__ __forceinline__ Eps edge(int lab, double b0, double bA, int E, double
   return r;
 }
 
+// Streaming (non-temporal) row accesses.  A pass writes each inside or
+// outside row once and the next level reads it once, so these accesses carry
+// the non-temporal hint and leave the caches to the per-site tables (priors,
+// site offsets, likelihoods) that every node of a tree reads.
+typedef double nt_double2 __attribute__((ext_vector_type(2)));
+typedef double nt_double4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ double2 load_stream(const double2* p) {
+  const nt_double2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_double2*>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void store_stream(double2* p, const double2& a) {
+  const nt_double2 v = {a.x, a.y};
+  __builtin_nontemporal_store(v, reinterpret_cast<nt_double2*>(p));
+}
+__device__ __forceinline__ void store_stream(double4* p, const double4& a) {
+  const nt_double4 v = {a.x, a.y, a.z, a.w};
+  __builtin_nontemporal_store(v, reinterpret_cast<nt_double4*>(p));
+}
+
 // binary exponent x of m (m in [2^(x-1), 2^x)); 0 for m == 0
 __device__ __forceinline__ int exp_of(double m) {
   int ex;

@@ -186,10 +186,11 @@ __device__ __forceinline__ void cherry_beta(const ChildRows& r, int k, double ph
 }
 
 // inside state of a child at site k (qo, qs: the site's prior offset and
-// mass, used for a cherry's leaves)
+// mass, used for a cherry's leaves).  stream: read the inside row with the
+// non-temporal hint (the inside pass reads each row once).
 __device__ __forceinline__ void load_child_rows(const ChildRows& r, int kind, int k, double phi,
                                                 const double* __restrict__ prior, int qo, double qs, int& lab,
-                                                double& b0, double& bA, int& E, int& code) {
+                                                double& b0, double& bA, int& E, int& code, bool stream = false) {
   if (kind == kLeaf) {
     code = row_at(r.code, k);
     leaf_beta(code, phi, lab, b0, bA);
@@ -200,7 +201,7 @@ __device__ __forceinline__ void load_child_rows(const ChildRows& r, int kind, in
   } else {
     code = 0;
     lab = row_at(r.code, k);
-    const double2 bb = row_at(r.beta, k);
+    const double2 bb = stream ? load_stream(&row_at(r.beta, k)) : row_at(r.beta, k);
     b0 = bb.x;
     bA = bb.y;
     E = row_at(r.bE, k);
@@ -223,6 +224,7 @@ struct InNode {
   const int32_t* qoff;   // prior offsets of the tree's sites
   const double* qsum;    // prior mass of the tree's sites
   double phi;
+  bool stream;           // rows with the non-temporal hint
 };
 
 // CH: the node may have a fused cherry child (otherwise children are leaves
@@ -244,6 +246,7 @@ __device__ __forceinline__ void in_node_load(const DevForest& F, int v, int t, i
   n.qoff = F.site_qoff + F.tree_site_base[t];
   n.qsum = F.site_qsum + F.tree_site_base[t];
   n.phi = F.phi[p];
+  n.stream = F.stream_rows != 0;
 }
 
 // The inside step of one site in three phases: row loads, prior gathers
@@ -261,8 +264,8 @@ __device__ __forceinline__ void inside_load(const DevForest& F, const InNode& n,
   // a missing ('?') child may be in any edit state: its prior weight is the
   // site's total prior mass
   x.qs = row_at(n.qsum, k);
-  load_child_rows(n.c[0], kind[0], k, n.phi, F.prior, x.qo, x.qs, x.la, x.a0, x.aA, x.Ea, x.ca);
-  load_child_rows(n.c[1], kind[1], k, n.phi, F.prior, x.qo, x.qs, x.lb, x.b0, x.bA, x.Eb, x.cb);
+  load_child_rows(n.c[0], kind[0], k, n.phi, F.prior, x.qo, x.qs, x.la, x.a0, x.aA, x.Ea, x.ca, n.stream);
+  load_child_rows(n.c[1], kind[1], k, n.phi, F.prior, x.qo, x.qs, x.lb, x.b0, x.bA, x.Eb, x.cb, n.stream);
 }
 
 __device__ __forceinline__ void inside_gather(const DevForest& F, InSite& x) {
@@ -279,7 +282,11 @@ __device__ __forceinline__ void inside_finish(const InNode& n, int k, const InSi
   // cannot produce an infinite scale)
   const int sh = min(-exp_of(fmax(B0, BA)), 1000);
   const double sc = ldexp(1.0, sh);
-  row_at(n.beta, k) = make_double2(B0 * sc, BA * sc);
+  const double2 b = make_double2(B0 * sc, BA * sc);
+  if (n.stream)
+    store_stream(&row_at(n.beta, k), b);
+  else
+    row_at(n.beta, k) = b;
   row_at(n.bE, k) = ea.e + eb.e - sh;
 }
 
@@ -498,6 +505,7 @@ struct OutNode {
   const int32_t* qoff;
   const double* qsum;
   double phi;
+  bool stream;           // rows with the non-temporal hint
 };
 
 template <bool CH>
@@ -525,6 +533,7 @@ __device__ __forceinline__ void out_node_load(const DevForest& F, int u, int t, 
   n.qoff = F.site_qoff + sb;
   n.qsum = F.site_qsum + sb;
   n.phi = F.phi[p];
+  n.stream = F.stream_rows != 0;
 }
 
 // The posterior step of one node at one site.  In: the node's outside state
@@ -668,7 +677,10 @@ __device__ __forceinline__ void outside_finish(const DevForest& F, const OutNode
               if (kind[i] == kCherry) {
                 cherry_post(F, n.c[i], k, a, ec, gg, iL, x.LE, x.qo, x.qs, n.phi, cacc[i], crt[i]);
               } else {
-                row_at(n.ca[i], k) = a;
+                if (n.stream)
+                  store_stream(&row_at(n.ca[i], k), a);
+                else
+                  row_at(n.ca[i], k) = a;
                 row_at(n.caE[i], k) = ec;
                 row_at(n.caG[i], k) = (int16_t)gg;
               }

@@ -25,6 +25,9 @@ struct DevForest {
   // (their parents form and consume their inside and outside states; no
   // cherry row is read or written); cherry_end == num_leaves: no fusion
   int32_t cherry_end;
+  // 1: the inside/outside rows of a pass exceed the last-level cache, and
+  // the row kernels read and write them with the non-temporal hint
+  int32_t stream_rows;
   double dmin, dmax;
   // topology (level order)
   const int32_t* level_off;  // [levels+1]
