@@ -148,6 +148,13 @@ __device__ __forceinline__ double dpp_mov(double x) {
   return __hiloint2double(hi, lo);
 }
 
+// the value of lane 63 of the wavefront, in every lane
+__device__ __forceinline__ double bcast_last(double x) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
+  return __hiloint2double(hi, lo);
+}
+
 // Sum over an aligned segment of W lanes (W = 8..64) with DPP: quad
 // permutations, row rotations and row broadcasts.  The sum is valid in the
 // LAST lane of each segment.

@@ -730,6 +730,8 @@ __device__ __forceinline__ void cherry_write_partials(const DevForest& F, int c,
 // segment sums of the partial counts; the last lane of the segment writes
 // them as partial number `slot` of each child branch (and of the node's
 // dropout sum), then clears partials slot+1 .. slot_end-1
+// (64-lane segments: the sums are broadcast from the last lane and each
+// child's slots [slot, slot_end) are written by one coalesced vector store)
 template <int W>
 __device__ __forceinline__ void out_write_partials(const DevForest& F, int u, const int (&ch)[2], int lane, int slot,
                                                    int slot_end, double (&acc)[2][5], double rt) {
@@ -738,6 +740,29 @@ __device__ __forceinline__ void out_write_partials(const DevForest& F, int u, co
 #pragma unroll
     for (int j = 0; j < 5; ++j) acc[i][j] = seg_sum_last<W>(acc[i][j]);
   rt = seg_sum_last<W>(rt);
+  if constexpr (W == 64) {
+    const int nv = 5 * (slot_end - slot);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      double s[5];
+#pragma unroll
+      for (int j = 0; j < 5; ++j) s[j] = bcast_last(acc[i][j]);
+      double* c = F.cpart + ((int64_t)ch[i] * F.max_chunks + slot) * 5;
+      for (int b = 0; b < nv; b += 64) {
+        const int x = b + lane;
+        if (x < nv) {
+          const double v = x == 0 ? s[0] : x == 1 ? s[1] : x == 2 ? s[2] : x == 3 ? s[3] : x == 4 ? s[4] : 0.0;
+          c[x] = v;
+        }
+      }
+    }
+    const double r = bcast_last(rt);
+    for (int b = 0; b < slot_end - slot; b += 64) {
+      const int x = b + lane;
+      if (x < slot_end - slot) F.rt_part[(int64_t)u * F.max_chunks + slot + x] = x == 0 ? r : 0.0;
+    }
+    return;
+  }
   if (lane == W - 1) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
