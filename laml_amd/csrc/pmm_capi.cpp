@@ -510,6 +510,9 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   UP(node_pos, D.node_pos.data(), N);
   UP(prob_blk_off, D.prob_blk_off.data(), P + 1);
   F.num_nu_blocks = D.prob_blk_off[P];
+  F.max_prob_nodes = 0;
+  for (int p = 0; p < P; ++p)
+    F.max_prob_nodes = std::max(F.max_prob_nodes, D.prob_node_off[p + 1] - D.prob_node_off[p]);
   UP(prob_flags, d->problem_flags, P);
   UP(prob_nobs, d->problem_nobs, P);
   UP(prob_phimin, d->problem_phimin, P);

@@ -1215,8 +1215,9 @@ __global__ void __launch_bounds__(kNuThreads) k_nu_eval(DevForest F, const doubl
       if (bb == 0.0) continue;
       const double dv = F.nu_d[i];
       const double en = expm1(x * dv);
-      g += dv * (bb / en - F.nu_a[i]);
-      h -= bb * dv * dv * (en + 1.0) / (en * en);
+      const double ri = 1.0 / en;
+      g += dv * fma(bb, ri, -F.nu_a[i]);
+      h -= bb * dv * dv * (en + 1.0) * (ri * ri);
     }
   }
   if (iter == 0 && (flags & 1u) && !rtilde_in) {
@@ -1293,6 +1294,84 @@ __global__ void __launch_bounds__(kNuThreads) k_nu_eval(DevForest F, const doubl
   st->x = xn;
   st->fin = fin;
   nu[p] = xn;
+}
+
+// The same phi update and nu solve with one block per problem and every
+// Newton evaluation in one launch (forests of several problems of up to
+// kNuSolveMaxNodes branches): the block sweeps the problem's branches per
+// evaluation and all threads take the same bracketed step.
+constexpr int kNuSolveThreads = 1024;
+constexpr int kNuSolveMaxNodes = 65536;
+
+__global__ void __launch_bounds__(kNuSolveThreads) k_nu_solve(DevForest F, const double* __restrict__ rtilde_in,
+                                                             double* __restrict__ nu, double* __restrict__ phi,
+                                                             const int32_t* __restrict__ done) {
+  __shared__ double sh[kNuSolveThreads / 64];
+  const int p = blockIdx.x;
+  if (done && done[p]) return;
+  const uint32_t flags = F.prob_flags[p];
+  const bool opt_nu = (flags & 2u) && F.prob_nfree[p] > 0;
+  const int b = F.prob_node_off[p], e = F.prob_node_off[p + 1];
+  if (flags & 1u) {
+    double r;
+    if (rtilde_in) {
+      r = rtilde_in[p];
+    } else {
+      const int C = (F.tree_nsites[F.prob_tree_off[p]] + F.chunk_w - 1) / F.chunk_w;
+      double rt = 0.0;
+      for (int i = F.prob_int_off[p] + threadIdx.x; i < e; i += kNuSolveThreads) {
+        const int64_t v = F.prob_nodes[i];
+        for (int c = 0; c < C; ++c) rt += F.rt_part[v * F.max_chunks + c];
+      }
+      r = block_sum<kNuSolveThreads>(rt, sh);
+    }
+    double phi_star = r / (F.prob_nobs[p] + r);
+    if (fabs(phi_star) < F.prob_phimin[p]) phi_star = 0.0;
+    if (threadIdx.x == 0) phi[p] = phi_star;
+  }
+  if (!opt_nu) return;
+  double x = (nu[p] > 0.0 && nu[p] < 1.0) ? nu[p] : 0.5;
+  double lo = 0.0, hi = 2.0;  // hi > 1: the bound x = 1 has not been evaluated
+  for (int iter = 0; iter <= kNuIters; ++iter) {
+    double g = 0.0, h = 0.0;
+    for (int i = b + threadIdx.x; i < e; i += kNuSolveThreads) {
+      const double bb = F.nu_b[i];
+      if (bb == 0.0) continue;
+      const double dv = F.nu_d[i];
+      const double en = expm1(x * dv);
+      const double ri = 1.0 / en;
+      g += dv * fma(bb, ri, -F.nu_a[i]);
+      h -= bb * dv * dv * (en + 1.0) * (ri * ri);
+    }
+    const double G = block_sum<kNuSolveThreads>(g, sh);
+    const double Hs = block_sum<kNuSolveThreads>(h, sh);
+    // bracketed Newton step on x h'(x), as in k_nu_eval
+    if (G > 0.0) lo = x; else if (G < 0.0) hi = x;
+    bool fin = false;
+    double xn;
+    if (G > 0.0 && x >= 1.0) {
+      xn = 1.0;
+      fin = true;
+    } else if (G == 0.0) {
+      xn = x;
+      fin = true;
+    } else {
+      const double step = x * G / (G + x * Hs);
+      const double top = fmin(hi, 1.0);
+      if (fabs(step) <= 1e-12 * x) {
+        xn = fmin(fmax(x - step, 0.0), 1.0);
+        fin = true;
+      } else {
+        xn = x - step;
+        if (xn >= 1.0 && hi > 1.0) xn = 1.0;
+        else if (!(xn > lo && xn < top)) xn = 0.5 * (lo + top);
+        fin = top - lo <= 1e-12 * top;
+      }
+    }
+    x = xn;
+    if (fin) break;
+  }
+  if (threadIdx.x == 0) nu[p] = x;
 }
 
 // ---------------------------------------------------------------------------
@@ -2142,6 +2221,12 @@ void run_mstep(const DevForest& F, const double* counts, const double* rtilde, d
                      U ? *U : none, U ? 1 : 0);
   if (U)
     hipLaunchKernelGGL(k_ultra, dim3(F.num_problems), dim3(kUltraThreads), 0, s, F, *U, d, nu, done);
+  // one launch when the problems keep enough blocks busy; otherwise split
+  // each problem's branches over several blocks per evaluation
+  if (F.max_prob_nodes <= kNuSolveMaxNodes && F.num_problems >= 8) {
+    hipLaunchKernelGGL(k_nu_solve, dim3(F.num_problems), dim3(kNuSolveThreads), 0, s, F, rtilde, nu, phi, done);
+    return;
+  }
   for (int it = 0; it <= kNuIters; ++it)
     hipLaunchKernelGGL(k_nu_eval, dim3(F.num_nu_blocks), dim3(kNuThreads), 0, s, F, rtilde, nu, phi, done, it);
 }

@@ -50,6 +50,7 @@ struct DevForest {
   const int32_t* node_pos;       // [N] position of each node in prob_nodes
   const int32_t* prob_blk_off;   // [P+1] blocks of the nu solve per problem
   int32_t num_nu_blocks;
+  int32_t max_prob_nodes;        // branches of the largest problem
   const uint32_t* prob_flags;
   const double* prob_nobs;
   const double* prob_phimin;
