@@ -160,6 +160,24 @@ int32_t laml_forest_pass_mode(laml_forest* f);
 int laml_forest_set_fusion(laml_forest* f, int32_t on);
 int32_t laml_forest_fusion(laml_forest* f);
 
+/* Heavy-path chains for the outside pass (environment LAML_CHAINS=1 turns the
+ * default on; laml_amd.autotune tries it): with row kernels (more than 32
+ * sites), full inside storage and no cherry fusion, the outside levels below
+ * the split (top_level, or the hybrid walk's first level) run as chains of
+ * nodes, each node followed by its child with more leaves below it; one
+ * wavefront walks a chain at a 64-site chunk, keeping that child's outside
+ * state in registers, so only the other children's outside rows are
+ * written and read.  Chains run in rounds (one launch each): a chain starts
+ * when the round before it has written its head's outside row.  Results are
+ * bit-identical to level launches with one chunk per wavefront.  Heavy
+ * children's outside rows are not stored: laml_posteriors and
+ * laml_quartet_em then recompute every outside row first (one outside pass
+ * in level launches).  laml_forest_chains reports whether the next outside
+ * pass uses chains (1) or not (0).
+ * No counterpart in the reference. */
+int laml_forest_set_chains(laml_forest* f, int32_t on);
+int32_t laml_forest_chains(laml_forest* f);
+
 int laml_forest_get_shapes(laml_forest* f, int32_t* inside_cpw, int32_t* outside_cpw, int32_t* top_level);
 int laml_forest_set_shapes(laml_forest* f, const int32_t* inside_cpw, const int32_t* outside_cpw,
                            int32_t top_level);

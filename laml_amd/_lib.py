@@ -100,6 +100,8 @@ SIGNATURES = {
     "laml_forest_pass_mode": (ctypes.c_int32, [ctypes.c_void_p]),
     "laml_forest_set_fusion": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "laml_forest_fusion": (ctypes.c_int32, [ctypes.c_void_p]),
+    "laml_forest_set_chains": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "laml_forest_chains": (ctypes.c_int32, [ctypes.c_void_p]),
     "laml_posteriors": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p, _f64p, ctypes.c_int64,
                                          ctypes.c_void_p]),
     "laml_quartet_em": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32] + [ctypes.c_void_p] * 6 +

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <tuple>
 #include <limits>
 #include <mutex>
 #include <cstdlib>
@@ -80,6 +81,14 @@ struct laml_forest {
   bool cherry_ok = false;
   bool inside_fused = false;
   laml::CherrySplit split;  // per-level node lists of the fused passes
+  // heavy-path chains for the outside levels below the split (requested;
+  // on by default for row kernels), the plans built so far by split level,
+  // and whether every internal node's outside row holds the last outside
+  // pass's state (chains keep heavy children's states in registers)
+  bool chains = false;
+  bool alpha_complete = true;
+  std::vector<laml::ChainPlan> chain_plans;
+  std::vector<int32_t> h_heavy;  // heavy child of each node (-1: no internal child)
   laml::TreeWalk TW{};
   // host copies for re-layout of the inside vectors
   std::vector<int32_t> h_parent;
@@ -595,6 +604,22 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   F.labels = labels;
   F.brow = F.row_off;
   f->h_parent.assign(d->parent, d->parent + N);
+  {
+    // heavy child: the internal child with more leaves below it (children
+    // precede their parents in level order)
+    std::vector<int32_t> nl(N, 1);
+    f->h_heavy.assign(N, -1);
+    for (int v = d->num_leaves; v < N; ++v) {
+      const int a = d->child0[v], b = d->child1[v];
+      nl[v] = nl[a] + nl[b];
+      const bool ia = a >= d->num_leaves, ib = b >= d->num_leaves;
+      if (ia && ib) f->h_heavy[v] = nl[b] > nl[a] ? b : a;
+      else if (ia) f->h_heavy[v] = a;
+      else if (ib) f->h_heavy[v] = b;
+    }
+    const char* env = getenv("LAML_CHAINS");
+    f->chains = env ? atoi(env) != 0 : false;
+  }
   f->h_row_off = row_off;
   f->h_K = stride;
   f->h_desc_off = D.row_off;
@@ -796,6 +821,16 @@ int laml_forest_set_fusion(laml_forest* f, int32_t on) {
 static bool fuse_now(const laml_forest* f);
 int32_t laml_forest_fusion(laml_forest* f) { return f ? (fuse_now(f) ? 1 : 0) : -1; }
 
+int laml_forest_set_chains(laml_forest* f, int32_t on) {
+  if (!f) return fail(LAML_EINVAL, "null forest");
+  f->chains = on != 0;
+  f->plan_version++;
+  return LAML_OK;
+}
+
+static bool chains_now(const laml_forest* f);
+int32_t laml_forest_chains(laml_forest* f) { return f ? (chains_now(f) ? 1 : 0) : -1; }
+
 int laml_forest_set_shapes(laml_forest* f, const int32_t* inside_cpw, const int32_t* outside_cpw, int32_t top_level) {
   if (!f) return fail(LAML_EINVAL, "null forest");
   const int H = (int)f->level_off.size() - 1;
@@ -933,6 +968,74 @@ static int hybrid_split(const laml_forest* f) {
   return std::max(1, std::min(f->h_top, H));
 }
 
+// whether the outside pass runs the levels below its split as heavy-path
+// chains (row kernels, full inside storage, no cherry fusion; the tree walk
+// pass has its own outside levels)
+static bool chains_now(const laml_forest* f) {
+  return f->chains && f->W == 64 && f->ckpt_L == 0 && f->pass_mode != LAML_PASS_TREES && !fuse_now(f);
+}
+
+// The chain plan of the internal nodes below level hs (built on first use,
+// kept for the forest's life).  Heads: tree roots below hs, nodes whose
+// parent lies at hs or above, and light children; a head's round is one
+// more than its parent's chain's (0 when the state comes from above).
+static const laml::ChainPlan* chain_plan(laml_forest* f, int hs) {
+  for (const laml::ChainPlan& cp : f->chain_plans)
+    if (cp.h_split == hs) return &cp;
+  const int N = f->F.num_nodes;
+  const int lo = f->level_off[1], B = f->level_off[hs];
+  std::vector<int32_t> chain_of(N, -1), round_of(N, 0);
+  std::vector<std::tuple<int, int, int>> heads;  // (round, -len, -head)
+  for (int v = B - 1; v >= lo; --v) {
+    const int pa = f->h_parent[v];
+    const bool above = pa < 0 || pa >= B;
+    if (!above && f->h_heavy[pa] == v) continue;
+    const int r = above ? 0 : round_of[chain_of[pa]] + 1;
+    int len = 0;
+    for (int w = v; w >= 0; w = f->h_heavy[w]) {
+      chain_of[w] = v;
+      ++len;
+    }
+    round_of[v] = r;
+    heads.emplace_back(r, -len, -v);
+  }
+  std::sort(heads.begin(), heads.end());
+  laml::ChainPlan cp;
+  cp.h_split = hs;
+  std::vector<int2> ch;
+  std::vector<int32_t> nodes;
+  ch.reserve(heads.size());
+  nodes.reserve(B - lo);
+  for (const auto& hd : heads) {
+    const int r = std::get<0>(hd);
+    while ((int)cp.round_off.size() <= r) cp.round_off.push_back((int32_t)ch.size());
+    ch.push_back(make_int2((int)nodes.size(), -std::get<1>(hd)));
+    for (int w = -std::get<2>(hd); w >= 0; w = f->h_heavy[w]) nodes.push_back(w);
+  }
+  cp.round_off.push_back((int32_t)ch.size());
+  if (f->upload(&cp.chains, ch.data(), ch.size()) || f->upload(&cp.nodes, nodes.data(), nodes.size()))
+    return nullptr;
+  f->chain_plans.push_back(std::move(cp));
+  return &f->chain_plans.back();
+}
+
+// builds the chain plan the next passes use, outside any stream capture
+static int prepare_chains(laml_forest* f) {
+  if (chains_now(f) && !chain_plan(f, hybrid_split(f))) return fail(LAML_ENOMEM, "chain plan");
+  return LAML_OK;
+}
+
+// outside levels [1, hs) of the level and hybrid passes
+static void outside_below(laml_forest* f, const DevForest& F, int hs, const int32_t* done, hipStream_t s,
+                          bool allow_chains) {
+  const laml::ChainPlan* cp = allow_chains && chains_now(f) ? chain_plan(f, hs) : nullptr;
+  if (cp)
+    laml::run_outside_chains(F, *cp, done, s);
+  else
+    laml::run_outside_levels(F, f->level_off, f->W, 1, hs, f->out_cpw, done, s);
+  f->alpha_complete = cp == nullptr;
+}
+
 // inside pass and root step (site likelihoods) in the forest's pass mode
 static void pass_inside_root(laml_forest* f, const DevForest& F, double* site_llh, const int32_t* done,
                              hipStream_t s) {
@@ -955,28 +1058,35 @@ static void pass_inside_root(laml_forest* f, const DevForest& F, double* site_ll
 // outside pass with posterior partials (and the root step first when asked);
 // after an inside pass with fused cherries the outside pass fuses them too
 // (the cherries' inside rows were not written)
-static void pass_outside(laml_forest* f, const DevForest& F, bool with_root, const int32_t* done, hipStream_t s) {
+static void pass_outside(laml_forest* f, const DevForest& F, bool with_root, const int32_t* done, hipStream_t s,
+                         bool allow_chains = true) {
   if (f->inside_fused) {
     const DevForest G = with_cherries(f, F, true);
     if (with_root) laml::run_root(G, f->W, nullptr, done, s);
     laml::run_outside(G, f->level_off, f->W, f->h_top, f->out_cpw, done, s, &f->split);
+    f->alpha_complete = false;
     return;
   }
   if (f->pass_mode == 1) {
     laml::run_tree_walk(F, f->TW, f->W, (with_root ? 2 : 0) | 4, nullptr, done, s);
+    f->alpha_complete = true;
     return;
   }
   if (f->pass_mode == LAML_PASS_HYBRID) {
     const int hs = hybrid_split(f);
     laml::run_tree_walk(F, f->TW, f->W, (with_root ? 2 : 0) | 4, nullptr, done, s, hs);
-    laml::run_outside_levels(F, f->level_off, f->W, 1, hs, f->out_cpw, done, s);
+    outside_below(f, F, hs, done, s, allow_chains);
     return;
   }
   if (with_root) laml::run_root(F, f->W, nullptr, done, s);
-  if (f->ckpt_L > 0)
+  if (f->ckpt_L > 0) {
     laml::run_outside_ckpt(F, f->level_off, f->W, f->ckpt_L, f->in_cpw, f->out_cpw, done, s);
-  else
-    laml::run_outside(F, f->level_off, f->W, f->h_top, f->out_cpw, done, s);
+    f->alpha_complete = true;
+    return;
+  }
+  const laml::ChainPlan* cp = allow_chains && chains_now(f) ? chain_plan(f, hybrid_split(f)) : nullptr;
+  laml::run_outside(F, f->level_off, f->W, f->h_top, f->out_cpw, done, s, nullptr, cp);
+  f->alpha_complete = cp == nullptr;
 }
 
 int laml_nll(laml_forest* f, const double* d, const double* nu, const double* phi, double* nll, double* site_llh,
@@ -1001,15 +1111,18 @@ int laml_estep(laml_forest* f, const double* d, const double* nu, const double* 
   DevForest F = f->F;
   F.phi = phi;
   laml::run_coef(F, d, nu, nullptr, s);
+  int rc = prepare_chains(f);
+  if (rc) return rc;
   if (f->pass_mode == 1) {
     f->inside_fused = false;
     laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, nullptr, s);
+    f->alpha_complete = true;
   } else if (f->pass_mode == LAML_PASS_HYBRID) {
     f->inside_fused = false;
     const int hs = hybrid_split(f);
     laml::run_inside_levels(F, f->level_off, f->W, 1, hs, f->in_cpw, nullptr, s);
     laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, nullptr, s, hs);
-    laml::run_outside_levels(F, f->level_off, f->W, 1, hs, f->out_cpw, nullptr, s);
+    outside_below(f, F, hs, nullptr, s, true);
   } else {
     pass_inside_root(f, F, nullptr, nullptr, s);
     pass_outside(f, F, false, nullptr, s);
@@ -1025,6 +1138,8 @@ int laml_outside(laml_forest* f, const double* phi, double* counts, double* rtil
   if (!f || !phi || !counts || !rtilde) return fail(LAML_EINVAL, "null argument");
   DeviceGuard guard(f->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc = prepare_chains(f);
+  if (rc) return rc;
   DevForest F = f->F;
   F.phi = phi;
   pass_outside(f, F, true, nullptr, s);
@@ -1038,6 +1153,8 @@ int laml_grad(laml_forest* f, const double* d, const double* nu, const double* p
   if (!f || !d || !nu || !phi || !grad_d || !grad_nu || !grad_phi) return fail(LAML_EINVAL, "null argument");
   DeviceGuard guard(f->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc = prepare_chains(f);
+  if (rc) return rc;
   DevForest F = f->F;
   F.phi = phi;
   pass_outside(f, F, true, nullptr, s);
@@ -1065,6 +1182,7 @@ static void em_step(laml_forest* f, const DevForest& F, const EmState& em, doubl
     // converges at this step is not used (its M-step is skipped)
     f->inside_fused = false;
     laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, em.done, s);
+    f->alpha_complete = true;
     laml::run_llh(F, nll, em, max_iter, tol, s);
   } else if (f->pass_mode == LAML_PASS_HYBRID && !last) {
     // bottom inside levels, one walk for the top of both passes, the
@@ -1074,7 +1192,7 @@ static void em_step(laml_forest* f, const DevForest& F, const EmState& em, doubl
     laml::run_inside_levels(F, f->level_off, f->W, 1, hs, f->in_cpw, em.done, s);
     laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, em.done, s, hs);
     laml::run_llh(F, nll, em, max_iter, tol, s);
-    laml::run_outside_levels(F, f->level_off, f->W, 1, hs, f->out_cpw, em.done, s);
+    outside_below(f, F, hs, em.done, s, true);
   } else {
     pass_inside_root(f, F, nullptr, em.done, s);
     laml::run_llh(F, nll, em, max_iter, tol, s);
@@ -1095,6 +1213,8 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
   const int P = F.num_problems;
   EmState em = f->em;
   em.trace = trace;
+  int prc = prepare_chains(f);
+  if (prc) return prc;
   laml::run_em_reset(em, P, s);
   // The EM step is captured once into a HIP graph and replayed: its kernel
   // arguments do not change between steps (the step number lives on the
@@ -1143,6 +1263,9 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
       if (all) break;
     }
   }
+  // a replayed step runs no host code: the outside rows are those of the
+  // captured step's pass
+  if (max_iter > 0 && f->pass_mode != LAML_PASS_TREES) f->alpha_complete = !chains_now(f) && !fuse_now(f);
   LAML_HIP(hipMemcpyAsync(iters, em.iters, sizeof(int32_t) * P, hipMemcpyDeviceToDevice, s));
   return check_launch();
 }
@@ -1160,6 +1283,10 @@ int laml_posteriors(laml_forest* f, const double* phi, double* post0_host, doubl
   hipStream_t s = static_cast<hipStream_t>(stream);
   DevForest F = f->F;
   F.phi = phi;
+  // the last outside pass kept heavy children's outside states in
+  // registers: form every node's outside row again from the resident inside
+  // state (one outside pass in level launches)
+  if (!f->alpha_complete) pass_outside(f, F, true, nullptr, s, false);
   const int64_t rows = f->leaf_rows + f->int_rows;
   double *p0 = nullptr, *pS = nullptr;
   LAML_HIP(hipMalloc(&p0, sizeof(double) * std::max<int64_t>(rows, 1)));
@@ -1201,6 +1328,11 @@ int laml_quartet_em(laml_forest* f, int32_t num_candidates, const int32_t* quart
                              "(laml_forest_set_fusion)");
   DeviceGuard guard(f->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!f->alpha_complete) {
+    DevForest F = f->F;
+    F.phi = phi;
+    pass_outside(f, F, true, nullptr, s, false);
+  }
   laml::run_quartet_em(f->F, num_candidates, quartet_nodes, lengths_in, optimise, llh_const, nu, phi, max_iter, tol,
                        lengths_out, nll, iters, s);
   return check_launch();

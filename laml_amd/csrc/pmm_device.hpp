@@ -148,6 +148,64 @@ __device__ __forceinline__ double dpp_mov(double x) {
   return __hiloint2double(hi, lo);
 }
 
+// 64-bit DPP move for patterns that write every lane (row rotations, quad
+// permutations): no old value to preserve
+template <int CTRL>
+__device__ __forceinline__ double dpp_all(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Lane swaps of two doubles: swap32 trades lanes 32..63 of x with lanes
+// 0..31 of y; swap16 trades the odd 16-lane rows of x with the even rows of
+// y (v_permlane32_swap / v_permlane16_swap).
+__device__ __forceinline__ void swap32(double& x, double& y) {
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y),
+                                                   false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y),
+                                                   false, false);
+  x = __hiloint2double((int)hi[0], (int)lo[0]);
+  y = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ __forceinline__ void swap16(double& x, double& y) {
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y),
+                                                   false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y),
+                                                   false, false);
+  x = __hiloint2double((int)hi[0], (int)lo[0]);
+  y = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+// Totals over the 64 lanes of a wavefront of twelve values v[0..11].  Two
+// butterfly steps pair lanes 32 and then 16 apart while halving the values
+// each lane carries (12 -> 6 -> 3), then rotations within each 16-lane row
+// finish the sums: on return every lane of row r (lanes 16r .. 16r + 15)
+// holds in v[0..2] the totals of values 3r .. 3r + 2.  About 60
+// instructions instead of 12 separate reductions.
+__device__ __forceinline__ void wave_sum12(double (&v)[12]) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    // lanes 0..31: value j of lanes L and L + 32; lanes 32..63: value j + 6
+    swap32(v[j], v[j + 6]);
+    v[j] += v[j + 6];
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    // rows 0 / 2 keep v[j], rows 1 / 3 take v[j + 3], each summed with the
+    // row 16 lanes away: row r now holds value j + 3r
+    swap16(v[j], v[j + 3]);
+    v[j] += v[j + 3];
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    v[j] += dpp_all<0x128>(v[j]);  // row_ror:8
+    v[j] += dpp_all<0x124>(v[j]);  // row_ror:4
+    v[j] += dpp_all<0x4e>(v[j]);   // quad_perm [2,3,0,1]
+    v[j] += dpp_all<0xb1>(v[j]);   // quad_perm [1,0,3,2]
+  }
+}
+
 // the value of lane 63 of the wavefront, in every lane
 __device__ __forceinline__ double bcast_last(double x) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);

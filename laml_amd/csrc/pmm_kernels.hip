@@ -637,9 +637,17 @@ struct OutSite {
   double b0[2], bA[2], q[2];
 };
 
+// the children's rows at site k (x.qo, x.qs already loaded)
+template <int K0, int K1>
+__device__ __forceinline__ void outside_load_children(const DevForest& F, const OutNode& n, int k, OutSite& x) {
+  const int kind[2] = {resolve_kind<K0>(n.kind[0]), resolve_kind<K1>(n.kind[1])};
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+    load_child_rows(n.c[i], kind[i], k, n.phi, F.prior, x.qo, x.qs, x.lab[i], x.b0[i], x.bA[i], x.E[i], x.code[i]);
+}
+
 template <int K0, int K1>
 __device__ __forceinline__ void outside_load(const DevForest& F, const OutNode& n, int k, OutSite& x) {
-  const int kind[2] = {resolve_kind<K0>(n.kind[0]), resolve_kind<K1>(n.kind[1])};
   x.au = row_at(n.a, k);
   x.Eo = row_at(n.aE, k);
   x.gu = row_at(n.aG, k);
@@ -647,9 +655,7 @@ __device__ __forceinline__ void outside_load(const DevForest& F, const OutNode& 
   x.LE = row_at(n.LE, k);
   x.qo = row_at(n.qoff, k) - 1;
   x.qs = row_at(n.qsum, k);
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-    load_child_rows(n.c[i], kind[i], k, n.phi, F.prior, x.qo, x.qs, x.lab[i], x.b0[i], x.bA[i], x.E[i], x.code[i]);
+  outside_load_children<K0, K1>(F, n, k, x);
 }
 
 __device__ __forceinline__ void outside_gather(const DevForest& F, OutSite& x) {
@@ -727,42 +733,54 @@ __device__ __forceinline__ void cherry_write_partials(const DevForest& F, int c,
   }
 }
 
-// segment sums of the partial counts; the last lane of the segment writes
-// them as partial number `slot` of each child branch (and of the node's
-// dropout sum), then clears partials slot+1 .. slot_end-1
-// (64-lane segments: the sums are broadcast from the last lane and each
-// child's slots [slot, slot_end) are written by one coalesced vector store)
+// segment sums of the partial counts; they are written as partial number
+// `slot` of each child branch (and of the node's dropout sum), and partials
+// slot+1 .. slot_end-1 are cleared.  64-lane segments: one butterfly sum of
+// all eleven values, each total stored by one lane, the clearing by
+// coalesced vector stores; narrower segments: DPP segment sums, stored by
+// the segment's last lane.
 template <int W>
 __device__ __forceinline__ void out_write_partials(const DevForest& F, int u, const int (&ch)[2], int lane, int slot,
                                                    int slot_end, double (&acc)[2][5], double rt) {
+  if constexpr (W == 64) {
+    double v[12];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      v[j] = acc[0][j];
+      v[5 + j] = acc[1][j];
+    }
+    v[10] = rt;
+    v[11] = 0.0;
+    wave_sum12(v);
+    // lane 16r + j (j < 3) stores total 3r + j: counts 0..4 of child 0,
+    // counts 0..4 of child 1, the dropout sum
+    const int r = lane >> 4, j = lane & 15;
+    const int vi = 3 * r + j;
+    if (j < 3 && vi < 11) {
+      const double x = j == 0 ? v[0] : (j == 1 ? v[1] : v[2]);
+      if (vi < 10) {
+        const int i = vi >= 5 ? 1 : 0;
+        F.cpart[((int64_t)ch[i] * F.max_chunks + slot) * 5 + (vi - 5 * i)] = x;
+      } else {
+        F.rt_part[(int64_t)u * F.max_chunks + slot] = x;
+      }
+    }
+    const int nz = slot_end - slot - 1;
+    if (nz > 0) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        double* c = F.cpart + ((int64_t)ch[i] * F.max_chunks + slot + 1) * 5;
+        for (int b = lane; b < 5 * nz; b += 64) c[b] = 0.0;
+      }
+      for (int b = lane; b < nz; b += 64) F.rt_part[(int64_t)u * F.max_chunks + slot + 1 + b] = 0.0;
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 5; ++j) acc[i][j] = seg_sum_last<W>(acc[i][j]);
   rt = seg_sum_last<W>(rt);
-  if constexpr (W == 64) {
-    const int nv = 5 * (slot_end - slot);
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      double s[5];
-#pragma unroll
-      for (int j = 0; j < 5; ++j) s[j] = bcast_last(acc[i][j]);
-      double* c = F.cpart + ((int64_t)ch[i] * F.max_chunks + slot) * 5;
-      for (int b = 0; b < nv; b += 64) {
-        const int x = b + lane;
-        if (x < nv) {
-          const double v = x == 0 ? s[0] : x == 1 ? s[1] : x == 2 ? s[2] : x == 3 ? s[3] : x == 4 ? s[4] : 0.0;
-          c[x] = v;
-        }
-      }
-    }
-    const double r = bcast_last(rt);
-    for (int b = 0; b < slot_end - slot; b += 64) {
-      const int x = b + lane;
-      if (x < slot_end - slot) F.rt_part[(int64_t)u * F.max_chunks + slot + x] = x == 0 ? r : 0.0;
-    }
-    return;
-  }
   if (lane == W - 1) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -880,6 +898,109 @@ __global__ void __launch_bounds__(256) k_outside_rows_ch(DevForest F, int beg, i
     outside_row_run<kLeaf, kCherry>(F, u, n, c0, c1, lane);
   } else {
     outside_row_run<kInternal, kCherry>(F, u, n, c0, c1, lane);
+  }
+}
+
+// Outside pass along heavy-path chains (row form, one 64-site chunk per
+// wavefront).  A chain is a node followed by its successive heavy children
+// (each the child with the larger subtree among the internal ones).  The
+// wavefront runs the posterior step of every node of its chain from the top,
+// keeping the heavy child's outside state in registers as the next node's
+// own: only light children's outside rows are written and only the head's
+// is read, which removes most of the outside-row traffic of the level
+// passes.  The site's likelihood and prior values are read once per chain.
+// Partial counts go to the chunk's slot as in the level kernels with one
+// chunk per wavefront, so the results are the same bits.
+template <int K0, int K1>
+__device__ __forceinline__ void chain_site(const DevForest& F, const OutNode& n, int k, OutSite& x, double iL,
+                                           int hv, double4& au, int& Eo, int& gu, double (&acc)[2][5],
+                                           double& rt) {
+  const int kind[2] = {resolve_kind<K0>(n.kind[0]), resolve_kind<K1>(n.kind[1])};
+  const bool lf[2] = {kind[0] == kLeaf, kind[1] == kLeaf};
+  x.au = au;
+  x.Eo = Eo;
+  x.gu = gu;
+  outside_load_children<K0, K1>(F, n, k, x);
+  outside_gather(F, x);
+  Eps ep[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) ep[i] = edge(x.lab[i], x.b0[i], x.bA[i], x.E[i], x.q[i], n.cf[i]);
+  post_site(x.au, x.Eo, x.gu, iL, x.LE, x.qs, n.phi, x.lab, x.b0, x.bA, x.E, x.code, x.q, ep, n.cf, lf, acc, rt,
+            [&](int i, const double4& a, int ec, int gg) {
+              if (i == hv) {
+                au = a;
+                Eo = ec;
+                gu = gg;
+              } else {
+                if (n.stream)
+                  store_stream(&row_at(n.ca[i], k), a);
+                else
+                  row_at(n.ca[i], k) = a;
+                row_at(n.caE[i], k) = ec;
+                row_at(n.caG[i], k) = (int16_t)gg;
+              }
+            });
+}
+
+// Chains [cbeg, cbeg + cnt) of a chain plan: chain c is
+// cnodes[chains[c].x .. chains[c].x + chains[c].y), head first.  Wavefront w
+// runs chain w / max_chunks at chunk w % max_chunks, so that every chunk of
+// the (longest, first) chains starts first.  Grid: ceil(cnt * max_chunks /
+// 4) blocks of 256; the head's outside row must be complete.
+__global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* __restrict__ chains, int cbeg,
+                                                        int cnt, const int32_t* __restrict__ cnodes,
+                                                        const int32_t* __restrict__ done) {
+  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  const int wv = w / F.max_chunks;
+  if (wv >= cnt) return;
+  const int2 cd = chains[cbeg + wv];
+  const int nb = __builtin_amdgcn_readfirstlane(cd.x);
+  const int len = __builtin_amdgcn_readfirstlane(cd.y);
+  int u = __builtin_amdgcn_readfirstlane(cnodes[nb]);
+  const int t = F.tree_of[u];
+  const int p = F.tree_prob[t];
+  const int K = F.tree_nsites[t];
+  const int chunk = w - wv * F.max_chunks;
+  if (chunk * 64 >= K) return;
+  if (done && done[p]) return;
+  const int lane = threadIdx.x & 63;
+  const int k = chunk * 64 + lane;
+  const bool on = k < K;
+  OutNode n;
+  out_node_load<false>(F, u, t, p, K, n);
+  OutSite x;
+  double iL = 0.0;
+  double4 au = make_double4(0.0, 0.0, 0.0, 0.0);
+  int Eo = 0, gu = 0;
+  if (on) {
+    au = row_at(n.a, k);
+    Eo = row_at(n.aE, k);
+    gu = row_at(n.aG, k);
+    x.Lm = row_at(n.Lm, k);
+    x.LE = row_at(n.LE, k);
+    x.qo = row_at(n.qoff, k) - 1;
+    x.qs = row_at(n.qsum, k);
+    iL = 1.0 / x.Lm;
+  }
+  for (int j = 0;;) {
+    const int nxt = j + 1 < len ? __builtin_amdgcn_readfirstlane(cnodes[nb + j + 1]) : -1;
+    const int hv = nxt < 0 ? -1 : (n.ch[0] == nxt ? 0 : 1);
+    double acc[2][5];
+    zero_acc(acc);
+    double rt = 0.0;
+    if (on) {
+      if (n.kind[0] == kLeaf) {
+        if (n.kind[1] == kLeaf) chain_site<kLeaf, kLeaf>(F, n, k, x, iL, hv, au, Eo, gu, acc, rt);
+        else chain_site<kLeaf, kInternal>(F, n, k, x, iL, hv, au, Eo, gu, acc, rt);
+      } else {
+        if (n.kind[1] == kLeaf) chain_site<kInternal, kLeaf>(F, n, k, x, iL, hv, au, Eo, gu, acc, rt);
+        else chain_site<kInternal, kInternal>(F, n, k, x, iL, hv, au, Eo, gu, acc, rt);
+      }
+    }
+    out_write_partials<64>(F, u, n.ch, lane, chunk, chunk + 1, acc, rt);
+    if (++j >= len) break;
+    u = nxt;
+    out_node_load<false>(F, u, t, p, K, n);
   }
 }
 
@@ -2082,7 +2203,8 @@ void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W
 }
 
 void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top,
-                 const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s, const CherrySplit* cs) {
+                 const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s, const CherrySplit* cs,
+                 const ChainPlan* cp) {
   const int H = (int)lvl.size() - 1;
   if (cs) {
     for (int h = H - 1; h >= 2; --h) split_level(true, F, *cs, cpw, h, done, s);
@@ -2094,7 +2216,21 @@ void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int
       hipLaunchKernelGGL(k_outside_top<WW>, dim3(F.max_chunks), dim3(1024), 0, s, F, ht, H, done);
     });
   }
+  if (cp) {
+    run_outside_chains(F, *cp, done, s);
+    return;
+  }
   for (int h = ht - 1; h >= 1; --h) outside_level(F, lvl, W, cpw, h, done, s);
+}
+
+void run_outside_chains(const DevForest& F, const ChainPlan& cp, const int32_t* done, hipStream_t s) {
+  for (size_t r = 0; r + 1 < cp.round_off.size(); ++r) {
+    const int beg = cp.round_off[r], cnt = cp.round_off[r + 1] - beg;
+    if (cnt <= 0) continue;
+    const int64_t waves = (int64_t)cnt * F.max_chunks;
+    hipLaunchKernelGGL(k_outside_chain, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, F, cp.chains, beg, cnt,
+                       cp.nodes, done);
+  }
 }
 
 void run_tree_walk(const DevForest& F, const TreeWalk& TW, int W, int phases, double* site_llh_out,

@@ -148,10 +148,27 @@ struct CherrySplit {
   std::vector<int32_t> off;
 };
 
+// Heavy-path chains of the outside pass over the levels below h_split
+// (k_outside_chain): chain c is nodes[chains[c].x .. chains[c].x +
+// chains[c].y), head first, each later node the heavy child of the one
+// before.  Round r holds chains [round_off[r], round_off[r + 1]), longest
+// first; the heads of round 0 get their outside state from the levels above
+// h_split or the root step, those of round r > 0 from round r - 1 (device
+// arrays, host offsets).
+struct ChainPlan {
+  int32_t h_split = -1;
+  const int2* chains = nullptr;
+  const int32_t* nodes = nullptr;
+  std::vector<int32_t> round_off;
+};
+
 void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const std::vector<int32_t>& cpw,
                 const int32_t* done, hipStream_t s, const CherrySplit* cs = nullptr);
+// cp: the levels below the top kernel's run as heavy-path chains
 void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top,
-                 const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s, const CherrySplit* cs = nullptr);
+                 const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s, const CherrySplit* cs = nullptr,
+                 const ChainPlan* cp = nullptr);
+void run_outside_chains(const DevForest& F, const ChainPlan& cp, const int32_t* done, hipStream_t s);
 void run_outside_ckpt(const DevForest& F, const std::vector<int32_t>& lvl, int W, int L,
                       const std::vector<int32_t>& in_cpw, const std::vector<int32_t>& out_cpw, const int32_t* done,
                       hipStream_t s);

@@ -525,6 +525,17 @@ class Forest(ForestLayout):
         """Whether the next level pass fuses the cherries."""
         return bool(self._lib.laml_forest_fusion(self.handle))
 
+    def set_chains(self, on):
+        """Heavy-path chains for the outside levels below the split: one
+        wavefront walks a node and its successive heavy children at a site
+        chunk, keeping their outside states in registers (same bits as level
+        launches with one chunk per wavefront)."""
+        _lib.check(self._lib.laml_forest_set_chains(self.handle, 1 if on else 0))
+
+    def chains(self):
+        """Whether the next outside pass runs heavy-path chains."""
+        return bool(self._lib.laml_forest_chains(self.handle))
+
     def get_shapes(self):
         """Launch shapes: (inside_cpw, outside_cpw, top_level); the cpw arrays
         hold, per level, the 64-site chunks one wavefront walks (all 0 when

@@ -1,0 +1,215 @@
+"""Heavy-path chains for the outside pass on the GPU: a wavefront walks a node
+and its successive heavy children at a site chunk, keeping the outside state
+in registers.  With one chunk per wavefront in the level launches the chain
+pass does the same operations in the same order, so the counts, dropout
+sums, likelihoods, EM trajectories and gradients are the same bits; the
+posteriors and quartet EM, which read every node's outside row, recompute
+them and agree bit for bit as well.  Cases: Yule trees of several sizes,
+ragged site counts (partial chunks), batches of trees with different site
+counts, a caterpillar (one chain as long as the tree), two-leaf trees, the
+level and hybrid modes with and without the top-level kernel, converged
+problems masked in the EM loop."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from laml_amd import sim
+from laml_amd.forest import Forest
+from oracle.em_oracle import OracleEM
+
+pytestmark = pytest.mark.gpu
+
+
+def _caterpillar(n, rng):
+    """((((l0, l1), l2), l3) ...) in post-order, random branch lengths."""
+    c0, c1, par = [], [], []
+
+    def node(a, b):
+        c0.append(a)
+        c1.append(b)
+        par.append(-1)
+        v = len(c0) - 1
+        if a >= 0:
+            par[a] = v
+            par[b] = v
+        return v
+
+    cur = node(-1, -1)
+    for _ in range(n - 1):
+        leaf = node(-1, -1)
+        cur = node(cur, leaf)
+    N = len(c0)
+    length = rng.uniform(0.01, 0.2, size=N)
+    nl = sum(1 for a in c0 if a < 0)
+    return sim.YuleTree(np.array(c0), np.array(c1), np.array(par), length, ["c%d" % i for i in range(nl)])
+
+
+def _problem(kind, n, K, seed):
+    if kind == "yule":
+        return sim.make_problem(n, K, 3, 20, 1.0, 0.1, 0.15, seed)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = _caterpillar(n, rng)
+    Q = sim.random_priors(K, 3, 20, 1.0, rng)
+    codes = sim.simulate_codes(t, Q, 0.1, 0.15, rng)
+    return t, Q, codes, None
+
+
+def _pair(problems, mode, top=None):
+    """The same forest twice, chains on and off, in pass mode `mode` with one
+    chunk per wavefront on every level (top: the split level; default all
+    levels in level launches)."""
+    specs = [sim.problem_spec(t, q, c) for (t, q, c, _) in problems]
+    dev = torch.cuda.current_device()
+    out = []
+    for on in (True, False):
+        F = Forest(specs, device=dev)
+        F.set_pass_mode(mode)
+        H = F.num_levels
+        F.set_shapes(np.ones(H, dtype=np.int32), np.ones(H, dtype=np.int32), H if top is None else min(top, H))
+        F.set_chains(on)
+        out.append(F)
+    rows = max(len(q) for (_, q, _, _) in problems) > 32
+    assert out[0].chains() == rows and not out[1].chains()
+    return out
+
+
+def _params(F, problems, nu=0.2, phi=0.1, start=None):
+    L = [t.length if start is None else np.full(t.n_nodes, start) for (t, _, _, _) in problems]
+    d = torch.tensor(F.lengths_to_level(L), dtype=torch.float64, device=F.device)
+    P = len(problems)
+    return (d, torch.full((P,), nu, dtype=torch.float64, device=F.device),
+            torch.full((P,), phi, dtype=torch.float64, device=F.device))
+
+
+def _estep(F, prm):
+    counts, rt, nll = F.estep(*prm)
+    return nll.cpu().numpy().copy(), counts.cpu().numpy().copy(), rt.cpu().numpy().copy()
+
+
+def _close(*Fs):
+    for F in Fs:
+        F.close()
+
+
+CASES = [  # (tree kind, leaves, sites, problems)
+    ("yule", 300, 150, 1), ("yule", 500, 200, 3), ("yule", 257, 65, 2), ("yule", 1200, 100, 2),
+    ("caterpillar", 400, 96, 1), ("yule", 2, 70, 3), ("yule", 150, 20, 1)]
+
+
+@pytest.mark.parametrize("kind,n,K,P", CASES)
+@pytest.mark.parametrize("mode,top", [(0, None), (0, 6), (2, 6), (2, None)])
+def test_chain_estep_equals_levels(kind, n, K, P, mode, top):
+    probs = [_problem(kind, n + 37 * i, K - 9 * i, 1300 + i) for i in range(P)]
+    print("case", kind, n, K, P, mode, top, flush=True)
+    Fa, Fb = _pair(probs, mode, top)
+    na, ca, ra = _estep(Fa, _params(Fa, probs))
+    nb, cb, rb = _estep(Fb, _params(Fb, probs))
+    np.testing.assert_array_equal(na, nb)
+    np.testing.assert_array_equal(ca, cb)
+    np.testing.assert_array_equal(ra, rb)
+    t, q, c, _ = probs[0]
+    o = OracleEM([t.newick()], sim.codes_to_charmtrx(t, c), q, nu=0.2, phi=0.1)
+    assert math.isclose(float(na[0]), o.negative_llh(), rel_tol=1e-9)
+    _close(Fa, Fb)
+
+
+def test_chain_em_graph_and_masking():
+    # problems of different sizes converge at different steps: the chain
+    # kernels skip masked problems like the level kernels
+    probs = [_problem("yule", 300 + 250 * i, 120 - 20 * i, 1400 + i) for i in range(4)]
+    for mode in (0, 2):
+        Fa, Fb = _pair(probs, mode, 8)
+        out = []
+        for F in (Fa, Fb):
+            d, nu, phi = _params(F, probs, nu=0.5, phi=0.5, start=0.05)
+            nll, it = F.em(d, nu, phi, 60, 1e-7)
+            out.append((nll.cpu().numpy().copy(), it.cpu().numpy().copy(), d.cpu().numpy().copy(),
+                        nu.cpu().numpy().copy(), phi.cpu().numpy().copy()))
+        for a, b in zip(out[0], out[1]):
+            np.testing.assert_array_equal(a, b)
+        print("mode", mode, "iters", out[0][1], flush=True)
+        # a second call replays the captured graph from the same start
+        d, nu, phi = _params(Fa, probs, nu=0.5, phi=0.5, start=0.05)
+        nll2, _ = Fa.em(d, nu, phi, 60, 1e-7)
+        np.testing.assert_array_equal(nll2.cpu().numpy(), out[0][0])
+        _close(Fa, Fb)
+
+
+def test_chain_gradient_equals_levels():
+    probs = [_problem("yule", 350, 100, 1500)]
+    Fa, Fb = _pair(probs, 2, 7)
+    g = []
+    for F in (Fa, Fb):
+        d, nu, phi = _params(F, probs)
+        F.nll(d, nu, phi)
+        gd, gnu, gphi = F.grad(d, nu, phi)
+        g.append((gd.cpu().numpy().copy(), gnu.item(), gphi.item()))
+    np.testing.assert_array_equal(g[0][0], g[1][0])
+    assert g[0][1] == g[1][1] and g[0][2] == g[1][2]
+    _close(Fa, Fb)
+
+
+def test_posteriors_after_chain_estep():
+    probs = [_problem("yule", 400, 90, 1600), _problem("yule", 250, 70, 1601)]
+    Fa, Fb = _pair(probs, 2, 6)
+    post = []
+    for F in (Fa, Fb):
+        prm = _params(F, probs)
+        F.estep(*prm)
+        p0, pS = F.posteriors(prm[2])
+        post.append((np.concatenate(p0), np.concatenate(pS)))
+    np.testing.assert_array_equal(post[0][0], post[1][0])
+    np.testing.assert_array_equal(post[0][1], post[1][1])
+    # the chained pass still runs after the recompute
+    assert Fa.chains()
+    na, ca, _ = _estep(Fa, _params(Fa, probs))
+    nb, cb, _ = _estep(Fb, _params(Fb, probs))
+    np.testing.assert_array_equal(ca, cb)
+    _close(Fa, Fb)
+
+
+def test_quartet_scores_with_chains(monkeypatch):
+    from laml_amd.quartet import QuartetScorer
+    from laml_amd.solvers import DEFAULT_STRATEGY, EM_solver
+    from laml_amd.topology_search import Topology_search_batched
+    import random
+
+    tree, Q, codes, _ = sim.make_problem(120, 80, 3, 15, 1.0, 0.1, 0.15, 17)
+    msa = sim.codes_to_charmtrx(tree, codes)
+    random.seed(17)
+    s = EM_solver([tree.newick()], {"charMtrx": msa}, {"Q": Q}, {"nu": 0, "phi": 0})
+    s.score_tree(strategy=dict(DEFAULT_STRATEGY, fixed_brlen=None))
+    res = []
+    for on in ("1", "0"):
+        monkeypatch.setenv("LAML_CHAINS", on)
+        ts = Topology_search_batched(s.get_tree_newick(), EM_solver, data={"charMtrx": msa}, prior={"Q": Q},
+                                     params=s.get_params())
+        random.seed(1)
+        moves = ts._list_moves()[::7]
+        scorer = QuartetScorer(ts, DEFAULT_STRATEGY, EM_solver)
+        sc = scorer.score(moves)
+        res.append(([x for x, _ in sc], [list(l.lengths) for _, l in sc]))
+        scorer.close()
+    assert res[0] == res[1]
+
+
+def test_chain_eligibility():
+    probs = [_problem("yule", 120, 70, 1700)]
+    Fa, Fb = _pair(probs, 0)
+    assert Fa.chains()
+    Fa.set_pass_mode(1)
+    assert not Fa.chains()
+    Fa.set_pass_mode(2)
+    assert Fa.chains()
+    Fa.set_checkpoint(2)
+    assert not Fa.chains()
+    Fa.set_checkpoint(0)
+    Fa.set_pass_mode(0)
+    assert Fa.chains()
+    Fa.set_fusion(True)
+    assert Fa.fusion() and not Fa.chains()
+    Fa.set_fusion(False)
+    assert Fa.chains()
+    _close(Fa, Fb)
