@@ -38,7 +38,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(F, fused=False):
+def heavy_children(F):
+    """Heavy child of every node in level order: the internal child with more
+    leaves below it (the first child on ties), -1 without an internal child
+    -- the chains of the outside pass follow these."""
+    N, n = F.num_nodes, F.num_leaves
+    nl = np.ones(N, dtype=np.int64)
+    heavy = np.full(N, -1, dtype=np.int64)
+    for h in range(1, F.num_levels):
+        v = np.arange(F.level_off[h], F.level_off[h + 1])
+        a, b = F.child0[v], F.child1[v]
+        nl[v] = nl[a] + nl[b]
+        ia, ib = a >= n, b >= n
+        heavy[v] = np.where(ia & ib, np.where(nl[b] > nl[a], b, a), np.where(ia, a, np.where(ib, b, -1)))
+    return heavy
+
+
+def algorithmic_bytes(F, fused=False, chain_split=None, in_chain_split=None):
     """Compulsory HBM bytes of one inside pass and one outside pass.
 
     Per (internal node, site): inside reads the node's label (2 B) and each
@@ -48,7 +64,12 @@ def algorithmic_bytes(F, fused=False):
     child's inside state again (2 / 22 B) and writes each internal child's
     outside state (38 B).  With fused cherries (nodes whose children are both
     leaves) a cherry has no rows: its parent reads the two leaf codes (4 B)
-    in place of its state, and the cherry's own steps move nothing else."""
+    in place of its state, and the cherry's own steps move nothing else.
+    With outside chains below level chain_split, a heavy child's outside
+    state stays in registers: it is neither written by its parent nor read by
+    itself (38 + 38 B less per heavy child and site); with inside chains
+    below level in_chain_split its parent does not read its inside state
+    (22 B less)."""
     N, n = F.num_nodes, F.num_leaves
     K = F.tree_K[F.tree_of].astype(np.int64)
     lo, hi = (int(F.level_off[1]), int(F.level_off[2])) if fused else (n, n)
@@ -57,9 +78,28 @@ def algorithmic_bytes(F, fused=False):
     cherry = (c >= lo) & (c < hi)
     rc = np.where(c < n, 2, np.where(cherry, 4, 22))
     wo = np.where((c < n) | cherry, 0, 38)
+    ra = np.full(len(inter), 38)
+    if chain_split is not None:
+        B = int(F.level_off[chain_split])
+        heavy = heavy_children(F)
+        below = inter < B
+        kept = (c == heavy[inter][None, :]) & below[None, :]
+        wo = np.where(kept, 0, wo)
+        # a node below the split reads its own state only as a chain head
+        par = np.full(N, -1, dtype=np.int64)
+        par[F.child0[inter]] = inter
+        par[F.child1[inter]] = inter
+        p = par[inter]
+        head = (p < 0) | (p >= B) | (heavy[np.maximum(p, 0)] != inter)
+        ra = np.where(below & ~head, 0, ra)
+    rci = rc
+    if in_chain_split is not None:
+        Bi = int(F.level_off[in_chain_split])
+        heavy = heavy_children(F)
+        rci = np.where((c == heavy[inter][None, :]) & (inter < Bi)[None, :], 0, rc)
     Kv = K[inter]
-    inside = int((Kv * (2 + 20 + rc.sum(0))).sum())
-    outside = int((Kv * (38 + rc.sum(0) + wo.sum(0))).sum())
+    inside = int((Kv * (2 + 20 + rci.sum(0))).sum())
+    outside = int((Kv * (ra + rc.sum(0) + wo.sum(0))).sum())
     return inside, outside
 
 
@@ -554,7 +594,10 @@ def main():
     # launches of k_outside, the fused top-level kernel, k_root, k_counts,
     # k_rtilde), timed with events on the launch stream over resident inside
     # state
-    inside_b, outside_b = algorithmic_bytes(F, fused=F.fusion())
+    split = min(max(1, int(F.get_shapes()[2])), F.num_levels)
+    flags = F.chains()
+    inside_b, outside_b = algorithmic_bytes(F, fused=F.fusion(), chain_split=split if flags & 1 else None,
+                                            in_chain_split=split if flags & 2 else None)
     F.nll(d, nu, phi)
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
@@ -603,7 +646,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": PROFILE if traffic is not None else None,
-                     "kernel": "outside pass (laml_outside: k_tree_walk top levels + root in the hybrid plan, k_outside_rows levels, k_counts, k_rtilde)",
+                     "kernel": "outside pass (laml_outside: k_tree_walk top levels + root in the hybrid plan, "
+                               "k_outside_chain rounds below the split when the plan has chains, else "
+                               "k_outside_rows levels, k_counts, k_rtilde)",
                      "algorithmic_bytes": outside_b, "ms": round(out_ms, 4),
                      "inside_pass": {"algorithmic_bytes": inside_b, "ms": round(in_ms, 4)}},
     }

@@ -160,22 +160,34 @@ int32_t laml_forest_pass_mode(laml_forest* f);
 int laml_forest_set_fusion(laml_forest* f, int32_t on);
 int32_t laml_forest_fusion(laml_forest* f);
 
-/* Heavy-path chains for the outside pass (environment LAML_CHAINS=1 turns the
- * default on; laml_amd.autotune tries it): with row kernels (more than 32
- * sites), full inside storage and no cherry fusion, the outside levels below
- * the split (top_level, or the hybrid walk's first level) run as chains of
- * nodes, each node followed by its child with more leaves below it; one
- * wavefront walks a chain at a 64-site chunk, keeping that child's outside
- * state in registers, so only the other children's outside rows are
- * written and read.  Chains run in rounds (one launch each): a chain starts
- * when the round before it has written its head's outside row.  Results are
- * bit-identical to level launches with one chunk per wavefront.  Heavy
- * children's outside rows are not stored: laml_posteriors and
- * laml_quartet_em then recompute every outside row first (one outside pass
- * in level launches).  laml_forest_chains reports whether the next outside
- * pass uses chains (1) or not (0).
+/* Heavy-path chains (environment LAML_CHAINS=<flags> sets the default flags;
+ * without it, outside chains are on for forests of at least 32,768
+ * (internal node, 64-site chunk) pairs; laml_amd.autotune tries them): with
+ * row kernels (more than 32 sites), full inside storage and no cherry
+ * fusion, the levels below a pass's chain-region top (and below the split:
+ * top_level, or the hybrid walk's first level) run as chains of nodes, each
+ * node followed by its child with more leaves below it; the levels from the
+ * region top to the split keep level launches.  One wavefront walks a chain
+ * at a 64-site chunk and keeps that child's state in registers:
+ * LAML_CHAINS_OUTSIDE, the outside pass walks chains top-down and writes and
+ * reads only the other children's outside rows; LAML_CHAINS_INSIDE, the
+ * inside pass walks them bottom-up and reads only the other children's
+ * inside rows (every inside row is still written).  Chains run in rounds
+ * (one launch each) ordered by their dependences.  Results are
+ * bit-identical to level launches with one chunk per wavefront.  After an
+ * outside pass with chains the heavy children's outside rows are not
+ * stored: laml_posteriors and laml_quartet_em then recompute every outside
+ * row first (one outside pass in level launches).
+ * which: LAML_CHAINS_* flags; inside_top / outside_top: chain-region top
+ * level of each pass (0: up to the split; defaults 5 and 10).  Measured on
+ * MI355X, outside chains below level 10 beat level launches on forests of
+ * that size; inside chains were slower on every forest tried (opt-in).
+ * laml_forest_chains reports the flags the next passes use.
  * No counterpart in the reference. */
-int laml_forest_set_chains(laml_forest* f, int32_t on);
+#define LAML_CHAINS_OUTSIDE 1
+#define LAML_CHAINS_INSIDE 2
+int laml_forest_set_chains(laml_forest* f, int32_t which, int32_t inside_top, int32_t outside_top);
+int laml_forest_get_chains(laml_forest* f, int32_t* which, int32_t* inside_top, int32_t* outside_top);
 int32_t laml_forest_chains(laml_forest* f);
 
 int laml_forest_get_shapes(laml_forest* f, int32_t* inside_cpw, int32_t* outside_cpw, int32_t* top_level);

@@ -15,6 +15,8 @@ LAML_OPT_NU = 2
 PASS_LEVELS = 0
 PASS_TREES = 1
 PASS_HYBRID = 2
+CHAINS_OUTSIDE = 1
+CHAINS_INSIDE = 2
 
 CODE_UNEDITED = 0
 CODE_MISSING = -1
@@ -100,7 +102,8 @@ SIGNATURES = {
     "laml_forest_pass_mode": (ctypes.c_int32, [ctypes.c_void_p]),
     "laml_forest_set_fusion": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "laml_forest_fusion": (ctypes.c_int32, [ctypes.c_void_p]),
-    "laml_forest_set_chains": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "laml_forest_set_chains": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "laml_forest_get_chains": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 3),
     "laml_forest_chains": (ctypes.c_int32, [ctypes.c_void_p]),
     "laml_posteriors": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p, _f64p, ctypes.c_int64,
                                          ctypes.c_void_p]),

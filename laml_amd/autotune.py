@@ -6,7 +6,7 @@ share one fused launch, and whether the cherries -- nodes with two leaf
 children -- are fused into their parents), as the tree walk (one launch per
 pass, a block per (tree, site chunk)), or as a hybrid of the two (level
 launches for the big bottom levels, one walk for the small top ones); the
-outside levels below the split can run as heavy-path chains.  Which is
+levels below the split can run as heavy-path chains in either pass.  Which is
 fastest depends on the forest: how
 many trees, how deep, how many sites, how uneven.  ``autotune`` times the
 E-step (inside pass, root, outside pass, counts) under candidate plans on
@@ -14,13 +14,13 @@ the forest itself and keeps the fastest; every plan computes the same values
 up to the order of floating-point sums over sites.
 
 The search is a coarse pass over pass mode (with a few hybrid splits),
-top-level fusion, cherry fusion and outside chains, then a coordinate search
+top-level fusion, cherry fusion and chains, then a coordinate search
 over the chunks per wavefront of the largest levels.
 """
 import numpy as np
 import torch
 
-from ._lib import PASS_HYBRID, PASS_LEVELS, PASS_TREES
+from ._lib import CHAINS_INSIDE, CHAINS_OUTSIDE, PASS_HYBRID, PASS_LEVELS, PASS_TREES
 
 
 def _time_estep(F, d, nu, phi, reps):
@@ -39,31 +39,35 @@ def _time_estep(F, d, nu, phi, reps):
 
 class Plan:
     """A launch plan: pass mode, per-level chunks per wavefront, top level,
-    cherry fusion, outside chains."""
+    cherry fusion, chain flags (CHAINS_OUTSIDE | CHAINS_INSIDE)."""
 
-    def __init__(self, mode, inside_cpw, outside_cpw, top_level, fuse=True, chains=False):
+    def __init__(self, mode, inside_cpw, outside_cpw, top_level, fuse=True, chains=0, chain_tops=(5, 10)):
         self.mode = int(mode)
+        self.chain_tops = tuple(int(x) for x in chain_tops)
         self.inside_cpw = np.asarray(inside_cpw, dtype=np.int32).copy()
         self.outside_cpw = np.asarray(outside_cpw, dtype=np.int32).copy()
         self.top_level = int(top_level)
         self.fuse = bool(fuse)
-        self.chains = bool(chains)
+        self.chains = int(chains)
 
     def copy(self):
-        return Plan(self.mode, self.inside_cpw, self.outside_cpw, self.top_level, self.fuse, self.chains)
+        return Plan(self.mode, self.inside_cpw, self.outside_cpw, self.top_level, self.fuse, self.chains,
+                    self.chain_tops)
 
     def apply(self, F):
         F.set_pass_mode(PASS_LEVELS)
         F.set_shapes(self.inside_cpw, self.outside_cpw, self.top_level)
         F.set_fusion(self.fuse)
-        F.set_chains(self.chains)
+        F.set_chains(self.chains, *self.chain_tops)
         F.set_pass_mode(self.mode)
 
     def describe(self):
         if self.mode == PASS_TREES:
             return {"mode": "trees"}
         return {"mode": "hybrid" if self.mode == PASS_HYBRID else "levels", "top_level": self.top_level,
-                "cherry_fusion": self.fuse, "outside_chains": self.chains, "inside_cpw": self.inside_cpw.tolist(),
+                "cherry_fusion": self.fuse, "outside_chains": bool(self.chains & CHAINS_OUTSIDE),
+                "inside_chains": bool(self.chains & CHAINS_INSIDE), "chain_tops": list(self.chain_tops),
+                "inside_cpw": self.inside_cpw.tolist(),
                 "outside_cpw": self.outside_cpw.tolist()}
 
 
@@ -72,11 +76,9 @@ def current_plan(F):
     mode = F.pass_mode()
     F.set_pass_mode(PASS_LEVELS)
     fuse = F.fusion()
-    F.set_fusion(False)
-    chains = F.chains()
-    F.set_fusion(fuse)
     F.set_pass_mode(mode)
-    return Plan(mode, a, b, top, fuse, chains)
+    chains, ti, to = F.get_chains()
+    return Plan(mode, a, b, top, fuse, chains, (ti, to))
 
 
 def autotune(F, d, nu, phi, reps=3, max_levels=8, verbose=False):
@@ -123,8 +125,8 @@ def autotune(F, d, nu, phi, reps=3, max_levels=8, verbose=False):
         if base.mode == PASS_HYBRID and h == base.top_level:
             continue
         cands.append(("hybrid from level %d" % h, Plan(PASS_HYBRID, base.inside_cpw, base.outside_cpw, h, False)))
-    # the same level and hybrid plans with the outside levels below the
-    # split as heavy-path chains (row kernels only)
+    # the same level and hybrid plans with and without outside chains (row
+    # kernels only)
     rows = base.inside_cpw[1:].max() > 0  # row kernels in use (more than 32 sites)
     if rows:
         chained = []
@@ -132,22 +134,47 @@ def autotune(F, d, nu, phi, reps=3, max_levels=8, verbose=False):
             if plan.mode == PASS_TREES or plan.fuse:
                 continue
             c = plan.copy()
-            c.chains = not plan.chains
-            chained.append(((tag + ", " if tag != "default" else "") + ("chains" if c.chains else "no chains"), c))
+            c.chains = plan.chains ^ CHAINS_OUTSIDE
+            chained.append(((tag + ", " if tag != "default" else "") +
+                            ("outside chains" if c.chains & CHAINS_OUTSIDE else "no outside chains"), c))
         cands += chained
     for tag, plan in cands:
         ms = measure(plan, tag)
         if ms < best_ms:
             best, best_ms = plan, ms
+    # the outside chain-region top, then inside chains on top of the plan
+    if rows and best.mode != PASS_TREES and not best.fuse:
+        trials = []
+        if best.chains & CHAINS_OUTSIDE:
+            trials += [(best.chains, (best.chain_tops[0], t)) for t in (0, 12, 10, 8, 6) if t != best.chain_tops[1]]
+        for tag_tops in trials:
+            trial = best.copy()
+            trial.chains, trial.chain_tops = tag_tops
+            ms = measure(trial, "chain_tops=%s" % (trial.chain_tops,))
+            if ms < best_ms * 0.995:
+                best, best_ms = trial, ms
+        for t in (4, 6):
+            trial = best.copy()
+            trial.chains = best.chains | CHAINS_INSIDE
+            trial.chain_tops = (t, best.chain_tops[1])
+            ms = measure(trial, "inside chains, top %d" % t)
+            if ms < best_ms * 0.995:
+                best, best_ms = trial, ms
     if best.mode in (PASS_LEVELS, PASS_HYBRID) and rows:
         sizes = np.diff(F.level_off)
         first = 2 if best.fuse else 1  # the cherry level has no launch of its own when fused
         big = [h for h in np.argsort(-sizes) if first <= h < min(H, best.top_level)][:max_levels]
         max_chunks = int(max(best.inside_cpw.max(), best.outside_cpw.max(), 1))
         options = sorted({1, 2, max(1, max_chunks // 2), max_chunks})
-        # chains run the outside levels below the split one chunk per wavefront
-        for which in ("inside_cpw",) if best.chains else ("outside_cpw", "inside_cpw"):
-            for h in big:
+        # a chained pass runs the levels below its chain-region top one
+        # chunk per wavefront; the levels above keep their launch shapes
+        split = min(H, best.top_level)
+        for which, bit, k in (("outside_cpw", CHAINS_OUTSIDE, 1), ("inside_cpw", CHAINS_INSIDE, 0)):
+            lo = 1
+            if best.chains & bit:
+                t = best.chain_tops[k]
+                lo = split if t <= 0 else min(t, split)
+            for h in [h for h in big if h >= lo]:
                 cur = int(getattr(best, which)[h])
                 for c in options:
                     if c == cur:

@@ -81,14 +81,19 @@ struct laml_forest {
   bool cherry_ok = false;
   bool inside_fused = false;
   laml::CherrySplit split;  // per-level node lists of the fused passes
-  // heavy-path chains for the outside levels below the split (requested;
-  // on by default for row kernels), the plans built so far by split level,
-  // and whether every internal node's outside row holds the last outside
-  // pass's state (chains keep heavy children's states in registers)
-  bool chains = false;
+  // heavy-path chains for the levels below the split (requested:
+  // LAML_CHAINS_OUTSIDE | LAML_CHAINS_INSIDE), the plans built so far by
+  // split level, and whether every internal node's outside row holds the
+  // last outside pass's state (outside chains keep heavy children's states
+  // in registers)
+  int chains = 0;
+  // top of the chain region of each pass (levels below min(top, split));
+  // 0: up to the split
+  int chain_top_in = 5, chain_top_out = 10;
   bool alpha_complete = true;
   std::vector<laml::ChainPlan> chain_plans;
   std::vector<int32_t> h_heavy;  // heavy child of each node (-1: no internal child)
+  std::vector<int32_t> h_child0, h_child1;
   laml::TreeWalk TW{};
   // host copies for re-layout of the inside vectors
   std::vector<int32_t> h_parent;
@@ -560,7 +565,7 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   int32_t *bE, *aE, *LE;
   double4* alpha;
   int16_t* aG;
-  double *Lm, *site_llh, *rt_part, *cpart, *nu_a, *nu_b, *nu_d, *nu_part;
+  double *iLm, *site_llh, *rt_part, *cpart, *nu_a, *nu_b, *nu_d, *nu_part;
   laml::NuState* nu_state;
   AL(labels, int16_t, int_rows);
   AL(coef, laml::Coef, N);
@@ -583,7 +588,7 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   AL(alpha, double4, int_rows);
   AL(aE, int32_t, int_rows);
   AL(aG, int16_t, int_rows);
-  AL(Lm, double, S);
+  AL(iLm, double, S);
   AL(LE, int32_t, S);
   AL(site_llh, double, S);
   AL(rt_part, double, (size_t)N * F.max_chunks);
@@ -609,6 +614,8 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
     // precede their parents in level order)
     std::vector<int32_t> nl(N, 1);
     f->h_heavy.assign(N, -1);
+    f->h_child0.assign(d->child0, d->child0 + N);
+    f->h_child1.assign(d->child1, d->child1 + N);
     for (int v = d->num_leaves; v < N; ++v) {
       const int a = d->child0[v], b = d->child1[v];
       nl[v] = nl[a] + nl[b];
@@ -617,8 +624,13 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
       else if (ia) f->h_heavy[v] = a;
       else if (ib) f->h_heavy[v] = b;
     }
+    // default: outside chains below level 10 for forests with enough
+    // (internal node, 64-site chunk) pairs to keep the chain rounds busy
+    // (32k; smaller forests are latency-bound and keep level launches)
     const char* env = getenv("LAML_CHAINS");
-    f->chains = env ? atoi(env) != 0 : false;
+    const int64_t units = (int64_t)(N - d->num_leaves) * ((max_k + 63) / 64);
+    f->chains = env ? (atoi(env) & (LAML_CHAINS_OUTSIDE | LAML_CHAINS_INSIDE))
+                    : (f->W == 64 && units >= 32768 ? LAML_CHAINS_OUTSIDE : 0);
   }
   f->h_row_off = row_off;
   f->h_K = stride;
@@ -633,7 +645,7 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   F.alpha = alpha;
   F.aE = aE;
   F.aG = aG;
-  F.Lm = Lm;
+  F.iLm = iLm;
   F.LE = LE;
   F.site_llh = site_llh;
   F.rt_part = rt_part;
@@ -821,15 +833,27 @@ int laml_forest_set_fusion(laml_forest* f, int32_t on) {
 static bool fuse_now(const laml_forest* f);
 int32_t laml_forest_fusion(laml_forest* f) { return f ? (fuse_now(f) ? 1 : 0) : -1; }
 
-int laml_forest_set_chains(laml_forest* f, int32_t on) {
+int laml_forest_set_chains(laml_forest* f, int32_t which, int32_t inside_top, int32_t outside_top) {
   if (!f) return fail(LAML_EINVAL, "null forest");
-  f->chains = on != 0;
+  if (which & ~(LAML_CHAINS_OUTSIDE | LAML_CHAINS_INSIDE)) return fail(LAML_EINVAL, "unknown chain flags");
+  if (inside_top < 0 || outside_top < 0) return fail(LAML_EINVAL, "negative chain region top");
+  f->chains = which;
+  f->chain_top_in = inside_top;
+  f->chain_top_out = outside_top;
   f->plan_version++;
   return LAML_OK;
 }
 
-static bool chains_now(const laml_forest* f);
-int32_t laml_forest_chains(laml_forest* f) { return f ? (chains_now(f) ? 1 : 0) : -1; }
+int laml_forest_get_chains(laml_forest* f, int32_t* which, int32_t* inside_top, int32_t* outside_top) {
+  if (!f || !which || !inside_top || !outside_top) return fail(LAML_EINVAL, "null argument");
+  *which = f->chains;
+  *inside_top = f->chain_top_in;
+  *outside_top = f->chain_top_out;
+  return LAML_OK;
+}
+
+static int chains_now(const laml_forest* f);
+int32_t laml_forest_chains(laml_forest* f) { return f ? chains_now(f) : -1; }
 
 int laml_forest_set_shapes(laml_forest* f, const int32_t* inside_cpw, const int32_t* outside_cpw, int32_t top_level) {
   if (!f) return fail(LAML_EINVAL, "null forest");
@@ -968,11 +992,11 @@ static int hybrid_split(const laml_forest* f) {
   return std::max(1, std::min(f->h_top, H));
 }
 
-// whether the outside pass runs the levels below its split as heavy-path
-// chains (row kernels, full inside storage, no cherry fusion; the tree walk
-// pass has its own outside levels)
-static bool chains_now(const laml_forest* f) {
-  return f->chains && f->W == 64 && f->ckpt_L == 0 && f->pass_mode != LAML_PASS_TREES && !fuse_now(f);
+// which passes run the levels below their split as heavy-path chains
+// (LAML_CHAINS_* bits; row kernels, full inside storage, no cherry fusion;
+// the tree walk pass has its own levels)
+static int chains_now(const laml_forest* f) {
+  return f->W == 64 && f->ckpt_L == 0 && f->pass_mode != LAML_PASS_TREES && !fuse_now(f) ? f->chains : 0;
 }
 
 // The chain plan of the internal nodes below level hs (built on first use,
@@ -1000,39 +1024,82 @@ static const laml::ChainPlan* chain_plan(laml_forest* f, int hs) {
     heads.emplace_back(r, -len, -v);
   }
   std::sort(heads.begin(), heads.end());
+  // inside rounds: children precede parents, so a light child's chain has
+  // its final round when its parent is visited
+  std::vector<int32_t> in_round(N, 0);
+  for (int v = lo; v < B; ++v) {
+    const int cs[2] = {f->h_child0[v], f->h_child1[v]};
+    for (int c : cs)
+      if (c >= f->F.num_leaves && c != f->h_heavy[v])
+        in_round[chain_of[v]] = std::max(in_round[chain_of[v]], in_round[c] + 1);
+  }
   laml::ChainPlan cp;
   cp.h_split = hs;
   std::vector<int2> ch;
-  std::vector<int32_t> nodes;
+  std::vector<int32_t> nodes, off_of(N, -1);
+  std::vector<std::tuple<int, int, int>> in_heads;  // (inside round, -len, -head)
   ch.reserve(heads.size());
   nodes.reserve(B - lo);
   for (const auto& hd : heads) {
-    const int r = std::get<0>(hd);
+    const int r = std::get<0>(hd), v = -std::get<2>(hd);
     while ((int)cp.round_off.size() <= r) cp.round_off.push_back((int32_t)ch.size());
+    off_of[v] = (int32_t)nodes.size();
     ch.push_back(make_int2((int)nodes.size(), -std::get<1>(hd)));
-    for (int w = -std::get<2>(hd); w >= 0; w = f->h_heavy[w]) nodes.push_back(w);
+    for (int w = v; w >= 0; w = f->h_heavy[w]) nodes.push_back(w);
+    in_heads.emplace_back(in_round[v], std::get<1>(hd), std::get<2>(hd));
   }
   cp.round_off.push_back((int32_t)ch.size());
-  if (f->upload(&cp.chains, ch.data(), ch.size()) || f->upload(&cp.nodes, nodes.data(), nodes.size()))
+  std::sort(in_heads.begin(), in_heads.end());
+  std::vector<int2> ich;
+  ich.reserve(in_heads.size());
+  for (const auto& hd : in_heads) {
+    const int r = std::get<0>(hd), v = -std::get<2>(hd);
+    while ((int)cp.in_round_off.size() <= r) cp.in_round_off.push_back((int32_t)ich.size());
+    ich.push_back(make_int2(off_of[v], -std::get<1>(hd)));
+  }
+  cp.in_round_off.push_back((int32_t)ich.size());
+  if (f->upload(&cp.chains, ch.data(), ch.size()) || f->upload(&cp.in_chains, ich.data(), ich.size()) ||
+      f->upload(&cp.nodes, nodes.data(), nodes.size()))
     return nullptr;
   f->chain_plans.push_back(std::move(cp));
   return &f->chain_plans.back();
 }
 
-// builds the chain plan the next passes use, outside any stream capture
+// the chain plan of a pass below split level hs (nullptr: that pass runs
+// no chains)
+static const laml::ChainPlan* pass_chains(laml_forest* f, bool inside, int hs) {
+  if (!(chains_now(f) & (inside ? LAML_CHAINS_INSIDE : LAML_CHAINS_OUTSIDE))) return nullptr;
+  const int t = inside ? f->chain_top_in : f->chain_top_out;
+  const int top = t <= 0 ? hs : std::min(t, hs);
+  return top > 1 ? chain_plan(f, top) : nullptr;
+}
+
+// builds the chain plans the next passes use, outside any stream capture
 static int prepare_chains(laml_forest* f) {
-  if (chains_now(f) && !chain_plan(f, hybrid_split(f))) return fail(LAML_ENOMEM, "chain plan");
+  const int hs = hybrid_split(f);
+  for (const bool inside : {true, false}) {
+    if (!(chains_now(f) & (inside ? LAML_CHAINS_INSIDE : LAML_CHAINS_OUTSIDE))) continue;
+    const int t = inside ? f->chain_top_in : f->chain_top_out;
+    const int top = t <= 0 ? hs : std::min(t, hs);
+    if (top > 1 && !chain_plan(f, top)) return fail(LAML_ENOMEM, "chain plan");
+  }
   return LAML_OK;
+}
+
+// inside levels [1, hs) of the level and hybrid passes: chains below the
+// chain region's top, level launches above it
+static void inside_below(laml_forest* f, const DevForest& F, int hs, const int32_t* done, hipStream_t s) {
+  const laml::ChainPlan* cp = pass_chains(f, true, hs);
+  if (cp) laml::run_inside_chains(F, *cp, done, s);
+  laml::run_inside_levels(F, f->level_off, f->W, cp ? cp->h_split : 1, hs, f->in_cpw, done, s);
 }
 
 // outside levels [1, hs) of the level and hybrid passes
 static void outside_below(laml_forest* f, const DevForest& F, int hs, const int32_t* done, hipStream_t s,
                           bool allow_chains) {
-  const laml::ChainPlan* cp = allow_chains && chains_now(f) ? chain_plan(f, hs) : nullptr;
-  if (cp)
-    laml::run_outside_chains(F, *cp, done, s);
-  else
-    laml::run_outside_levels(F, f->level_off, f->W, 1, hs, f->out_cpw, done, s);
+  const laml::ChainPlan* cp = allow_chains ? pass_chains(f, false, hs) : nullptr;
+  laml::run_outside_levels(F, f->level_off, f->W, cp ? cp->h_split : 1, hs, f->out_cpw, done, s);
+  if (cp) laml::run_outside_chains(F, *cp, done, s);
   f->alpha_complete = cp == nullptr;
 }
 
@@ -1042,7 +1109,7 @@ static void pass_inside_root(laml_forest* f, const DevForest& F, double* site_ll
   if (f->pass_mode == LAML_PASS_HYBRID) {
     f->inside_fused = false;
     const int hs = hybrid_split(f);
-    laml::run_inside_levels(F, f->level_off, f->W, 1, hs, f->in_cpw, done, s);
+    inside_below(f, F, hs, done, s);
     laml::run_tree_walk(F, f->TW, f->W, 3, site_llh, done, s, hs);
   } else if (f->pass_mode == 1) {
     f->inside_fused = false;
@@ -1050,7 +1117,8 @@ static void pass_inside_root(laml_forest* f, const DevForest& F, double* site_ll
   } else {
     f->inside_fused = fuse_now(f);
     const DevForest G = with_cherries(f, F, f->inside_fused);
-    laml::run_inside(G, f->level_off, f->W, f->h_top, f->in_cpw, done, s, f->inside_fused ? &f->split : nullptr);
+    const laml::ChainPlan* cp = pass_chains(f, true, hybrid_split(f));
+    laml::run_inside(G, f->level_off, f->W, f->h_top, f->in_cpw, done, s, f->inside_fused ? &f->split : nullptr, cp);
     laml::run_root(G, f->W, site_llh, done, s);
   }
 }
@@ -1084,7 +1152,7 @@ static void pass_outside(laml_forest* f, const DevForest& F, bool with_root, con
     f->alpha_complete = true;
     return;
   }
-  const laml::ChainPlan* cp = allow_chains && chains_now(f) ? chain_plan(f, hybrid_split(f)) : nullptr;
+  const laml::ChainPlan* cp = allow_chains ? pass_chains(f, false, hybrid_split(f)) : nullptr;
   laml::run_outside(F, f->level_off, f->W, f->h_top, f->out_cpw, done, s, nullptr, cp);
   f->alpha_complete = cp == nullptr;
 }
@@ -1094,6 +1162,8 @@ int laml_nll(laml_forest* f, const double* d, const double* nu, const double* ph
   if (!f || !d || !nu || !phi || !nll) return fail(LAML_EINVAL, "null argument");
   DeviceGuard guard(f->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc = prepare_chains(f);
+  if (rc) return rc;
   DevForest F = f->F;
   F.phi = phi;
   laml::run_coef(F, d, nu, nullptr, s);
@@ -1120,7 +1190,7 @@ int laml_estep(laml_forest* f, const double* d, const double* nu, const double* 
   } else if (f->pass_mode == LAML_PASS_HYBRID) {
     f->inside_fused = false;
     const int hs = hybrid_split(f);
-    laml::run_inside_levels(F, f->level_off, f->W, 1, hs, f->in_cpw, nullptr, s);
+    inside_below(f, F, hs, nullptr, s);
     laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, nullptr, s, hs);
     outside_below(f, F, hs, nullptr, s, true);
   } else {
@@ -1189,7 +1259,7 @@ static void em_step(laml_forest* f, const DevForest& F, const EmState& em, doubl
     // convergence test, then the bottom outside levels
     f->inside_fused = false;
     const int hs = hybrid_split(f);
-    laml::run_inside_levels(F, f->level_off, f->W, 1, hs, f->in_cpw, em.done, s);
+    inside_below(f, F, hs, em.done, s);
     laml::run_tree_walk(F, f->TW, f->W, 7, nullptr, em.done, s, hs);
     laml::run_llh(F, nll, em, max_iter, tol, s);
     outside_below(f, F, hs, em.done, s, true);
@@ -1265,7 +1335,8 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
   }
   // a replayed step runs no host code: the outside rows are those of the
   // captured step's pass
-  if (max_iter > 0 && f->pass_mode != LAML_PASS_TREES) f->alpha_complete = !chains_now(f) && !fuse_now(f);
+  if (max_iter > 0 && f->pass_mode != LAML_PASS_TREES)
+    f->alpha_complete = !pass_chains(f, false, hybrid_split(f)) && !fuse_now(f);
   LAML_HIP(hipMemcpyAsync(iters, em.iters, sizeof(int32_t) * P, hipMemcpyDeviceToDevice, s));
   return check_launch();
 }

@@ -394,6 +394,114 @@ __global__ void __launch_bounds__(256) k_inside_rows_ch(DevForest F, int beg, in
   }
 }
 
+// Inside pass along heavy-path chains (row form, one 64-site chunk per
+// wavefront): the wavefront walks its chain from the deepest node (whose
+// children are both leaves) up to the head, keeping each node's inside
+// state in registers for its parent, so only light children's inside rows
+// are read.  Every node's inside row is still written (the outside pass
+// reads it).  The same operations as the level kernels, the same bits.
+struct InCarry {
+  int lab, E;
+  double b0, bA;
+};
+
+// one site of node n whose child HV (0 or 1) is the carried heavy child and
+// whose other child has kind KL; HV < 0: both children are leaves
+template <int HV, int KL>
+__device__ __forceinline__ void chain_in_site(const DevForest& F, const InNode& n, int k, int qo, double qs,
+                                              InCarry& c) {
+  InSite x;
+  x.lv = row_at(n.lab, k);
+  x.qo = qo;
+  x.qs = qs;
+  if (HV == 0) {
+    x.la = c.lab;
+    x.a0 = c.b0;
+    x.aA = c.bA;
+    x.Ea = c.E;
+    load_child_rows(n.c[1], KL, k, n.phi, F.prior, qo, qs, x.lb, x.b0, x.bA, x.Eb, x.cb, n.stream);
+  } else if (HV == 1) {
+    load_child_rows(n.c[0], KL, k, n.phi, F.prior, qo, qs, x.la, x.a0, x.aA, x.Ea, x.ca, n.stream);
+    x.lb = c.lab;
+    x.b0 = c.b0;
+    x.bA = c.bA;
+    x.Eb = c.E;
+  } else {
+    load_child_rows(n.c[0], kLeaf, k, n.phi, F.prior, qo, qs, x.la, x.a0, x.aA, x.Ea, x.ca, n.stream);
+    load_child_rows(n.c[1], kLeaf, k, n.phi, F.prior, qo, qs, x.lb, x.b0, x.bA, x.Eb, x.cb, n.stream);
+  }
+  inside_gather(F, x);
+  const Eps ea = edge(x.la, x.a0, x.aA, x.Ea, x.qa, n.cf[0]);
+  const Eps eb = edge(x.lb, x.b0, x.bA, x.Eb, x.qb, n.cf[1]);
+  const double B0 = ea.e0 * eb.e0;
+  const double BA = x.lv != 0 ? ea.eA * eb.eA : 0.0;
+  const int sh = min(-exp_of(fmax(B0, BA)), 1000);
+  const double sc = ldexp(1.0, sh);
+  const double2 b = make_double2(B0 * sc, BA * sc);
+  if (n.stream)
+    store_stream(&row_at(n.beta, k), b);
+  else
+    row_at(n.beta, k) = b;
+  const int E = ea.e + eb.e - sh;
+  row_at(n.bE, k) = E;
+  c.lab = x.lv;
+  c.b0 = b.x;
+  c.bA = b.y;
+  c.E = E;
+}
+
+// Chains [cbeg, cbeg + cnt) of an inside chain order (chain c is
+// cnodes[chains[c].x .. chains[c].x + chains[c].y), head first; the walk
+// starts at the last node).  Wavefront w runs chain w / max_chunks at chunk
+// w % max_chunks; light children's inside rows must be complete.
+__global__ void __launch_bounds__(256) k_inside_chain(DevForest F, const int2* __restrict__ chains, int cbeg,
+                                                       int cnt, const int32_t* __restrict__ cnodes,
+                                                       const int32_t* __restrict__ done) {
+  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  const int wv = w / F.max_chunks;
+  if (wv >= cnt) return;
+  const int2 cd = chains[cbeg + wv];
+  const int nb = __builtin_amdgcn_readfirstlane(cd.x);
+  const int len = __builtin_amdgcn_readfirstlane(cd.y);
+  int prev = -1;
+  int v = __builtin_amdgcn_readfirstlane(cnodes[nb + len - 1]);
+  const int t = F.tree_of[v];
+  const int p = F.tree_prob[t];
+  const int K = F.tree_nsites[t];
+  const int chunk = w - wv * F.max_chunks;
+  if (chunk * 64 >= K) return;
+  if (done && done[p]) return;
+  const int k = chunk * 64 + (int)(threadIdx.x & 63);
+  const bool on = k < K;
+  const int sb = F.tree_site_base[t];
+  int qo = 0;
+  double qs = 0.0;
+  if (on) {
+    qo = F.site_qoff[sb + k] - 1;
+    qs = F.site_qsum[sb + k];
+  }
+  InCarry c{0, 0, 0.0, 0.0};
+  for (int j = len - 1;; --j) {
+    InNode n;
+    in_node_load<false>(F, v, t, p, K, n);
+    const int hv = prev < 0 ? -1 : (F.child0[v] == prev ? 0 : 1);
+    if (on) {
+      if (hv < 0) {
+        chain_in_site<-1, kLeaf>(F, n, k, qo, qs, c);
+      } else if (hv == 0) {
+        if (n.kind[1] == kLeaf) chain_in_site<0, kLeaf>(F, n, k, qo, qs, c);
+        else chain_in_site<0, kInternal>(F, n, k, qo, qs, c);
+      } else {
+        if (n.kind[0] == kLeaf) chain_in_site<1, kLeaf>(F, n, k, qo, qs, c);
+        else chain_in_site<1, kInternal>(F, n, k, qo, qs, c);
+      }
+    }
+    if (j == 0) break;
+    prev = v;
+    v = __builtin_amdgcn_readfirstlane(cnodes[nb + j - 1]);
+  }
+}
+
 // Inside pass over the top levels [h0, h1): one block per site chunk walks
 // the levels in order (its segments share each level's nodes), so the small
 // upper levels cost one launch instead of one each.
@@ -437,7 +545,7 @@ __device__ __forceinline__ void root_chunk(const DevForest& F, int t, int chunk,
     const int ex = exp_of(er.e0);
     const double Lm = ldexp(er.e0, -ex);
     const int LE = er.e + ex;
-    F.Lm[sb + k] = Lm;
+    F.iLm[sb + k] = 1.0 / Lm;
     F.LE[sb + k] = LE;
     const double sl = log(Lm) + (double)LE * 0.69314718055994530942;
     F.site_llh[sb + k] = sl;
@@ -500,7 +608,7 @@ struct OutNode {
   const double4* a;      // the node's outside row
   const int32_t* aE;
   const int16_t* aG;
-  const double* Lm;      // the tree's site likelihoods (mantissa, exponent)
+  const double* iLm;     // the tree's site likelihoods (1 / mantissa, exponent)
   const int32_t* LE;
   const int32_t* qoff;
   const double* qsum;
@@ -528,7 +636,7 @@ __device__ __forceinline__ void out_node_load(const DevForest& F, int u, int t, 
   n.aE = F.aE + ru;
   n.aG = F.aG + ru;
   const int sb = F.tree_site_base[t];
-  n.Lm = F.Lm + sb;
+  n.iLm = F.iLm + sb;
   n.LE = F.LE + sb;
   n.qoff = F.site_qoff + sb;
   n.qsum = F.site_qsum + sb;
@@ -632,7 +740,7 @@ __device__ __forceinline__ void cherry_post(const DevForest& F, const ChildRows&
 struct OutSite {
   double4 au;
   int Eo, gu, LE, qo;
-  double Lm, qs;
+  double iL, qs;
   int lab[2], E[2], code[2];
   double b0[2], bA[2], q[2];
 };
@@ -651,7 +759,7 @@ __device__ __forceinline__ void outside_load(const DevForest& F, const OutNode& 
   x.au = row_at(n.a, k);
   x.Eo = row_at(n.aE, k);
   x.gu = row_at(n.aG, k);
-  x.Lm = row_at(n.Lm, k);
+  x.iL = row_at(n.iLm, k);
   x.LE = row_at(n.LE, k);
   x.qo = row_at(n.qoff, k) - 1;
   x.qs = row_at(n.qsum, k);
@@ -674,7 +782,7 @@ __device__ __forceinline__ void outside_finish(const DevForest& F, const OutNode
                                                double (&crt)[2]) {
   const int kind[2] = {resolve_kind<K0>(n.kind[0]), resolve_kind<K1>(n.kind[1])};
   const bool lf[2] = {kind[0] == kLeaf, kind[1] == kLeaf};
-  const double iL = 1.0 / x.Lm;
+  const double iL = x.iL;
   Eps ep[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) ep[i] = edge(x.lab[i], x.b0[i], x.bA[i], x.E[i], x.q[i], n.cf[i]);
@@ -976,11 +1084,11 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
     au = row_at(n.a, k);
     Eo = row_at(n.aE, k);
     gu = row_at(n.aG, k);
-    x.Lm = row_at(n.Lm, k);
+    x.iL = row_at(n.iLm, k);
     x.LE = row_at(n.LE, k);
     x.qo = row_at(n.qoff, k) - 1;
     x.qs = row_at(n.qsum, k);
-    iL = 1.0 / x.Lm;
+    iL = x.iL;
   }
   for (int j = 0;;) {
     const int nxt = j + 1 < len ? __builtin_amdgcn_readfirstlane(cnodes[nb + j + 1]) : -1;
@@ -2164,7 +2272,7 @@ static void split_level(bool outside, const DevForest& F, const CherrySplit& cs,
 }
 
 void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const std::vector<int32_t>& cpw,
-                const int32_t* done, hipStream_t s, const CherrySplit* cs) {
+                const int32_t* done, hipStream_t s, const CherrySplit* cs, const ChainPlan* cp) {
   const int H = (int)lvl.size() - 1;
   if (cs) {
     // fused cherries: row kernels on every level from 2 up, no top kernel
@@ -2172,7 +2280,10 @@ void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int 
     return;
   }
   const int ht = std::max(1, std::min(h_top, H));
-  for (int h = 1; h < ht; ++h) inside_level(F, lvl, W, cpw, h, done, s);
+  // chains below the chain plan's top level, level launches from there
+  const int h0 = cp ? std::min(cp->h_split, ht) : 1;
+  if (cp) run_inside_chains(F, *cp, done, s);
+  for (int h = h0; h < ht; ++h) inside_level(F, lvl, W, cpw, h, done, s);
   if (ht < H) {
     LAML_DISPATCH_W(W, {
       hipLaunchKernelGGL(k_inside_top<WW>, dim3(F.max_chunks), dim3(1024), 0, s, F, ht, H, done);
@@ -2216,11 +2327,19 @@ void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int
       hipLaunchKernelGGL(k_outside_top<WW>, dim3(F.max_chunks), dim3(1024), 0, s, F, ht, H, done);
     });
   }
-  if (cp) {
-    run_outside_chains(F, *cp, done, s);
-    return;
+  const int h0 = cp ? std::min(cp->h_split, ht) : 1;
+  for (int h = ht - 1; h >= h0; --h) outside_level(F, lvl, W, cpw, h, done, s);
+  if (cp) run_outside_chains(F, *cp, done, s);
+}
+
+void run_inside_chains(const DevForest& F, const ChainPlan& cp, const int32_t* done, hipStream_t s) {
+  for (size_t r = 0; r + 1 < cp.in_round_off.size(); ++r) {
+    const int beg = cp.in_round_off[r], cnt = cp.in_round_off[r + 1] - beg;
+    if (cnt <= 0) continue;
+    const int64_t waves = (int64_t)cnt * F.max_chunks;
+    hipLaunchKernelGGL(k_inside_chain, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, F, cp.in_chains, beg, cnt,
+                       cp.nodes, done);
   }
-  for (int h = ht - 1; h >= 1; --h) outside_level(F, lvl, W, cpw, h, done, s);
 }
 
 void run_outside_chains(const DevForest& F, const ChainPlan& cp, const int32_t* done, hipStream_t s) {
@@ -2270,7 +2389,7 @@ __global__ void __launch_bounds__(256) k_node_post(DevForest F, int64_t leaf_row
   if (k >= K) return;
   const int sb = F.tree_site_base[t];
   const double phi = F.phi[prob];
-  const double iL = 1.0 / F.Lm[sb + k];
+  const double iL = F.iLm[sb + k];
   const int LE = F.LE[sb + k];
   const int64_t ru = F.row_off[u];
   const double4 a4 = F.alpha[ru + k];

@@ -70,7 +70,7 @@ struct DevForest {
   double4* alpha;  // internal rows
   int32_t* aE;
   int16_t* aG;
-  double* Lm;  // [sites]
+  double* iLm;  // [sites] 1 / mantissa of the site likelihood (L = 2^LE / iLm)
   int32_t* LE;
   double* site_llh;
   double* rt_part;  // [N * max_chunks] dropout sums of leaf children per chunk
@@ -148,22 +148,28 @@ struct CherrySplit {
   std::vector<int32_t> off;
 };
 
-// Heavy-path chains of the outside pass over the levels below h_split
-// (k_outside_chain): chain c is nodes[chains[c].x .. chains[c].x +
+// Heavy-path chains over the levels below h_split (k_outside_chain,
+// k_inside_chain): chain c is nodes[chains[c].x .. chains[c].x +
 // chains[c].y), head first, each later node the heavy child of the one
-// before.  Round r holds chains [round_off[r], round_off[r + 1]), longest
-// first; the heads of round 0 get their outside state from the levels above
-// h_split or the root step, those of round r > 0 from round r - 1 (device
-// arrays, host offsets).
+// before.  Outside round r holds chains [round_off[r], round_off[r + 1]),
+// longest first; the heads of round 0 get their outside state from the
+// levels above h_split or the root step, those of round r > 0 from round
+// r - 1.  in_chains lists the same chains in inside rounds: a chain runs
+// one round after the last chain hanging off it (its light internal
+// children's), round 0 holding chains with none (device arrays, host
+// offsets).
 struct ChainPlan {
   int32_t h_split = -1;
   const int2* chains = nullptr;
+  const int2* in_chains = nullptr;
   const int32_t* nodes = nullptr;
-  std::vector<int32_t> round_off;
+  std::vector<int32_t> round_off, in_round_off;
 };
 
+// cp: the levels below the top kernel's run as heavy-path chains
 void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const std::vector<int32_t>& cpw,
-                const int32_t* done, hipStream_t s, const CherrySplit* cs = nullptr);
+                const int32_t* done, hipStream_t s, const CherrySplit* cs = nullptr, const ChainPlan* cp = nullptr);
+void run_inside_chains(const DevForest& F, const ChainPlan& cp, const int32_t* done, hipStream_t s);
 // cp: the levels below the top kernel's run as heavy-path chains
 void run_outside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top,
                  const std::vector<int32_t>& cpw, const int32_t* done, hipStream_t s, const CherrySplit* cs = nullptr,

@@ -525,16 +525,29 @@ class Forest(ForestLayout):
         """Whether the next level pass fuses the cherries."""
         return bool(self._lib.laml_forest_fusion(self.handle))
 
-    def set_chains(self, on):
-        """Heavy-path chains for the outside levels below the split: one
-        wavefront walks a node and its successive heavy children at a site
-        chunk, keeping their outside states in registers (same bits as level
-        launches with one chunk per wavefront)."""
-        _lib.check(self._lib.laml_forest_set_chains(self.handle, 1 if on else 0))
+    def set_chains(self, which, inside_top=None, outside_top=None):
+        """Heavy-path chains for the levels below a chain-region top (and the
+        split): one wavefront walks a node and its successive heavy children
+        at a site chunk, keeping their states in registers (same bits as
+        level launches with one chunk per wavefront).  which: True (both
+        passes), False, or the flags CHAINS_OUTSIDE | CHAINS_INSIDE;
+        inside_top / outside_top: region top level per pass (0: up to the
+        split; None keeps the current value)."""
+        flags = (_lib.CHAINS_OUTSIDE | _lib.CHAINS_INSIDE) if which is True else int(which)
+        _, ti, to = self.get_chains()
+        ti = ti if inside_top is None else int(inside_top)
+        to = to if outside_top is None else int(outside_top)
+        _lib.check(self._lib.laml_forest_set_chains(self.handle, flags, ti, to))
+
+    def get_chains(self):
+        """(requested flags, inside region top, outside region top)."""
+        w, ti, to = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(self._lib.laml_forest_get_chains(self.handle, ctypes.byref(w), ctypes.byref(ti), ctypes.byref(to)))
+        return w.value, ti.value, to.value
 
     def chains(self):
-        """Whether the next outside pass runs heavy-path chains."""
-        return bool(self._lib.laml_forest_chains(self.handle))
+        """The chain flags the next passes use (0: none)."""
+        return int(self._lib.laml_forest_chains(self.handle))
 
     def get_shapes(self):
         """Launch shapes: (inside_cpw, outside_cpw, top_level); the cpw arrays

@@ -56,22 +56,24 @@ def _problem(kind, n, K, seed):
     return t, Q, codes, None
 
 
-def _pair(problems, mode, top=None):
-    """The same forest twice, chains on and off, in pass mode `mode` with one
-    chunk per wavefront on every level (top: the split level; default all
-    levels in level launches)."""
+def _pair(problems, mode, top=None, which=True, tops=(0, 0)):
+    """The same forest twice, chains on (both passes, or the flags `which`,
+    with chain-region tops `tops` = (inside, outside), 0: up to the split)
+    and off, in pass mode `mode` with one chunk per wavefront on every level
+    (top: the split level; default all levels in level launches)."""
     specs = [sim.problem_spec(t, q, c) for (t, q, c, _) in problems]
     dev = torch.cuda.current_device()
     out = []
-    for on in (True, False):
+    for on in (which, False):
         F = Forest(specs, device=dev)
         F.set_pass_mode(mode)
         H = F.num_levels
         F.set_shapes(np.ones(H, dtype=np.int32), np.ones(H, dtype=np.int32), H if top is None else min(top, H))
-        F.set_chains(on)
+        F.set_chains(on, *tops)
         out.append(F)
     rows = max(len(q) for (_, q, _, _) in problems) > 32
-    assert out[0].chains() == rows and not out[1].chains()
+    want = (3 if which is True else which) if rows else 0
+    assert out[0].chains() == want and out[1].chains() == 0
     return out
 
 
@@ -99,11 +101,12 @@ CASES = [  # (tree kind, leaves, sites, problems)
 
 
 @pytest.mark.parametrize("kind,n,K,P", CASES)
-@pytest.mark.parametrize("mode,top", [(0, None), (0, 6), (2, 6), (2, None)])
-def test_chain_estep_equals_levels(kind, n, K, P, mode, top):
+@pytest.mark.parametrize("mode,top,which,tops", [(0, None, 3, (0, 0)), (0, 6, 1, (0, 3)), (2, 6, 3, (4, 0)),
+                                                  (2, None, 2, (0, 0)), (0, 7, 2, (3, 5)), (2, 9, 3, (5, 6))])
+def test_chain_estep_equals_levels(kind, n, K, P, mode, top, which, tops):
     probs = [_problem(kind, n + 37 * i, K - 9 * i, 1300 + i) for i in range(P)]
-    print("case", kind, n, K, P, mode, top, flush=True)
-    Fa, Fb = _pair(probs, mode, top)
+    print("case", kind, n, K, P, mode, top, which, tops, flush=True)
+    Fa, Fb = _pair(probs, mode, top, which, tops)
     na, ca, ra = _estep(Fa, _params(Fa, probs))
     nb, cb, rb = _estep(Fb, _params(Fb, probs))
     np.testing.assert_array_equal(na, nb)
@@ -115,12 +118,25 @@ def test_chain_estep_equals_levels(kind, n, K, P, mode, top):
     _close(Fa, Fb)
 
 
+def test_inside_chain_nll_and_site_likelihoods():
+    probs = [_problem("yule", 700, 130, 1450), _problem("caterpillar", 300, 70, 1451)]
+    Fa, Fb = _pair(probs, 2, 9, 2)
+    out = []
+    for F in (Fa, Fb):
+        d, nu, phi = _params(F, probs)
+        nll = F.nll(d, nu, phi, site_llh=True)
+        out.append((nll.cpu().numpy().copy(), F.site_llh.cpu().numpy().copy()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    _close(Fa, Fb)
+
+
 def test_chain_em_graph_and_masking():
     # problems of different sizes converge at different steps: the chain
     # kernels skip masked problems like the level kernels
     probs = [_problem("yule", 300 + 250 * i, 120 - 20 * i, 1400 + i) for i in range(4)]
     for mode in (0, 2):
-        Fa, Fb = _pair(probs, mode, 8)
+        Fa, Fb = _pair(probs, mode, 8, True, (4, 6))
         out = []
         for F in (Fa, Fb):
             d, nu, phi = _params(F, probs, nu=0.5, phi=0.5, start=0.05)
@@ -198,7 +214,15 @@ def test_quartet_scores_with_chains(monkeypatch):
 def test_chain_eligibility():
     probs = [_problem("yule", 120, 70, 1700)]
     Fa, Fb = _pair(probs, 0)
-    assert Fa.chains()
+    assert Fa.chains() == 3
+    Fa.set_chains(1)
+    assert Fa.chains() == 1
+    with pytest.raises(Exception):
+        Fa.set_chains(4)
+    with pytest.raises(Exception):
+        Fa.set_chains(1, -1, 0)
+    Fa.set_chains(True, 3, 4)
+    assert Fa.get_chains() == (3, 3, 4)
     Fa.set_pass_mode(1)
     assert not Fa.chains()
     Fa.set_pass_mode(2)

@@ -197,3 +197,42 @@ def test_ultrametric_start_heights_are_strictly_feasible():
             depth.append(acc)
         assert max(depth) - min(depth) < 1e-9
     assert bad_plain > 0  # the case the scaled placement exists for does occur
+
+
+def test_chain_byte_accounting():
+    """The benchmark's algorithmic bytes with outside chains: every heavy
+    child below the split saves its outside-row write and read (76 B per
+    site); without a chain region the count is the level pass's."""
+    import bench
+
+    probs = [sim.make_problem(300 + 50 * i, 40, 3, 10, 1.0, 0.1, 0.1, 60 + i) for i in range(3)]
+    F = ForestLayout([sim.problem_spec(t, q, c) for (t, q, c, _) in probs])
+    base = bench.algorithmic_bytes(F)
+    assert bench.algorithmic_bytes(F, chain_split=1) == base
+    n, N = F.num_leaves, F.num_nodes
+    # leaf counts by recursion from each root
+    nl = {}
+
+    def count(v):
+        if v < n:
+            return 1
+        if v not in nl:
+            nl[v] = count(int(F.child0[v])) + count(int(F.child1[v]))
+        return nl[v]
+
+    heavy = bench.heavy_children(F)
+    K = F.tree_K[F.tree_of]
+    for v in range(n, N):
+        a, b = int(F.child0[v]), int(F.child1[v])
+        want = [c for c in (a, b) if c >= n]
+        if len(want) == 2:
+            want = [b] if count(b) > count(a) else [a]
+        assert heavy[v] == (want[0] if want else -1)
+    H = F.num_levels
+    for split in (H, 5, 2):
+        B = int(F.level_off[split])
+        saved = 76 * sum(int(K[heavy[v]]) for v in range(n, B) if heavy[v] >= 0)
+        ins, out = bench.algorithmic_bytes(F, chain_split=split)
+        assert ins == base[0] and out == base[1] - saved
+        ins, out = bench.algorithmic_bytes(F, in_chain_split=split)
+        assert out == base[1] and ins == base[0] - saved // 76 * 22
