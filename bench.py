@@ -109,13 +109,17 @@ PROFILE = "profiles/outside_pass_em10k_batch16.md"
 def outside_traffic_from_profile(args):
     """HBM bytes of one outside pass of this workload, as measured with
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy's
-    counts) and recorded in the committed profile summary; None if absent."""
+    counts), and the vector-ALU issue fraction of the pass (SQ_INSTS_VALU,
+    GRBM_GUI_ACTIVE), as recorded in the committed profile summary; None
+    where absent."""
     import re
     path = os.path.join(ROOT, PROFILE)
     if not os.path.exists(path):
-        return None
-    m = re.search(r"measured traffic total \| ([\d.]+) ", open(path).read())
-    return int(float(m.group(1)) * 1e9) if m else None
+        return None, None
+    txt = open(path).read()
+    m = re.search(r"measured traffic total \| ([\d.]+) ", txt)
+    v = re.search(r"valu issue fraction of the pass \| ([\d.]+)", txt)
+    return (int(float(m.group(1)) * 1e9) if m else None), (float(v.group(1)) if v else None)
 
 
 def cpu_baseline_python(tree, Q, codes, target_s=4.0):
@@ -517,6 +521,10 @@ def main():
     ap.add_argument("--batch", type=int, default=16, help="independent problems per step")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch plan of the batch forest (no autotuning before timing)")
+    ap.add_argument("--save-plan", default=None,
+                    help="write the batch forest's autotuned launch plan to this JSON file")
+    ap.add_argument("--plan", default=None,
+                    help="apply the launch plan in this JSON file (written by --save-plan) instead of autotuning")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the measurement, run 3 plain device copies of this many MiB (a known-size "
                          "reference for hardware byte counters)")
@@ -558,13 +566,22 @@ def main():
         d = torch.tensor(d0, dtype=torch.float64, device=dev)
         nu = torch.full((len(problems),), 0.5, dtype=torch.float64, device=dev)
         phi = torch.full((len(problems),), 0.5, dtype=torch.float64, device=dev)
-        if tune:
+        if tune and args.plan:
+            from laml_amd.autotune import plan_from_dict
+            plan = plan_from_dict(json.load(open(args.plan)), F)
+            plan.apply(F)
+            plan_desc.update(plan.describe())
+            log("launch plan from %s: %s" % (args.plan, plan.describe().get("mode")))
+        elif tune:
             from laml_amd.autotune import autotune
             t1 = time.perf_counter()
             plan, tlog = autotune(F, d, nu, phi)
             plan_desc.update(plan.describe())
             log("autotuned launch plan in %.2f s: %s (E-step %.3f ms -> %.3f ms)" % (
                 time.perf_counter() - t1, plan.describe().get("mode"), tlog[0][1], min(ms for _, ms in tlog)))
+            if args.save_plan:
+                with open(args.save_plan, "w") as fh:
+                    json.dump(plan.describe(), fh)
         if warmup > 0:
             F.em(d, nu, phi, warmup, 0.0)
         torch.cuda.synchronize()
@@ -620,7 +637,7 @@ def main():
         out_ms, achieved, in_ms, inside_b / (in_ms * 1e-3) / 1e9))
 
     node_site = 2 * F.num_nodes * K  # over the whole batch
-    traffic = outside_traffic_from_profile(args) if B == 16 and args.workload == "em10k" else None
+    traffic, valu = outside_traffic_from_profile(args) if B == 16 and args.workload == "em10k" else (None, None)
     res = {
         "metric": "EM iters/s at 10k cells x 300 sites",
         "value": round(em_per_s, 3),
@@ -646,6 +663,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": PROFILE if traffic is not None else None,
+                     "valu_issue_frac": valu,
                      "kernel": "outside pass (laml_outside: k_tree_walk top levels + root in the hybrid plan, "
                                "k_outside_chain rounds below the split when the plan has chains, else "
                                "k_outside_rows levels, k_counts, k_rtilde)",

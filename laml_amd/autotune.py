@@ -71,6 +71,18 @@ class Plan:
                 "outside_cpw": self.outside_cpw.tolist()}
 
 
+def plan_from_dict(d, F):
+    """The plan a ``Plan.describe()`` dictionary names, for forest F (for
+    replaying a tuned plan on another run of the same forest)."""
+    if d["mode"] == "trees":
+        base = current_plan(F)
+        return Plan(PASS_TREES, base.inside_cpw, base.outside_cpw, base.top_level, False)
+    mode = PASS_HYBRID if d["mode"] == "hybrid" else PASS_LEVELS
+    chains = (CHAINS_OUTSIDE if d.get("outside_chains") else 0) | (CHAINS_INSIDE if d.get("inside_chains") else 0)
+    return Plan(mode, d["inside_cpw"], d["outside_cpw"], d["top_level"], d.get("cherry_fusion", False), chains,
+                d.get("chain_tops", (5, 10)))
+
+
 def current_plan(F):
     a, b, top = F.get_shapes()
     mode = F.pass_mode()

@@ -1058,8 +1058,20 @@ static const laml::ChainPlan* chain_plan(laml_forest* f, int hs) {
     ich.push_back(make_int2(off_of[v], -std::get<1>(hd)));
   }
   cp.in_round_off.push_back((int32_t)ich.size());
+  std::vector<laml::ChainStep> steps(nodes.size());
+  for (size_t i = 0; i < nodes.size(); ++i) {
+    const int v = nodes[i];
+    laml::ChainStep& r = steps[i];
+    r.u = v;
+    r.ch0 = f->h_child0[v];
+    r.ch1 = f->h_child1[v];
+    const int hv = f->h_heavy[v];
+    r.hv = hv < 0 ? -1 : (hv == r.ch0 ? 0 : 1);
+    r.ro0 = f->h_row_off[r.ch0];
+    r.ro1 = f->h_row_off[r.ch1];
+  }
   if (f->upload(&cp.chains, ch.data(), ch.size()) || f->upload(&cp.in_chains, ich.data(), ich.size()) ||
-      f->upload(&cp.nodes, nodes.data(), nodes.size()))
+      f->upload(&cp.nodes, nodes.data(), nodes.size()) || f->upload(&cp.steps, steps.data(), steps.size()))
     return nullptr;
   f->chain_plans.push_back(std::move(cp));
   return &f->chain_plans.back();

@@ -1050,13 +1050,35 @@ __device__ __forceinline__ void chain_site(const DevForest& F, const OutNode& n,
             });
 }
 
-// Chains [cbeg, cbeg + cnt) of a chain plan: chain c is
-// cnodes[chains[c].x .. chains[c].x + chains[c].y), head first.  Wavefront w
+// The per-node values of one chain step, from its record: children, their
+// kinds and rows (no level checkpointing: inside rows sit at row_off), the
+// branch coefficients; tree-wide values (phi, site pointers) stay as set.
+__device__ __forceinline__ void chain_step_node(const DevForest& F, const ChainStep& r, OutNode& n) {
+  const int ch[2] = {r.ch0, r.ch1};
+  const int64_t ro[2] = {r.ro0, r.ro1};
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    n.ch[i] = ch[i];
+    n.kind[i] = ch[i] < F.num_leaves ? kLeaf : kInternal;
+    n.cf[i] = F.coef[ch[i]];
+    n.c[i].code = (n.kind[i] == kLeaf ? F.codes : F.labels) + ro[i];
+    n.c[i].beta = F.beta + ro[i];
+    n.c[i].bE = F.bE + ro[i];
+    n.ca[i] = F.alpha + ro[i];
+    n.caE[i] = F.aE + ro[i];
+    n.caG[i] = F.aG + ro[i];
+  }
+}
+
+// Chains [cbeg, cbeg + cnt) of a chain plan: chain c's steps are
+// steps[chains[c].x .. chains[c].x + chains[c].y), head first.  Wavefront w
 // runs chain w / max_chunks at chunk w % max_chunks, so that every chunk of
-// the (longest, first) chains starts first.  Grid: ceil(cnt * max_chunks /
-// 4) blocks of 256; the head's outside row must be complete.
+// the (longest, first) chains starts first.  Each step's record is loaded
+// one step ahead, so a step's row loads need no dependent metadata loads.
+// Grid: ceil(cnt * max_chunks / 4) blocks of 256; the head's outside row
+// must be complete.
 __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* __restrict__ chains, int cbeg,
-                                                        int cnt, const int32_t* __restrict__ cnodes,
+                                                        int cnt, const ChainStep* __restrict__ steps,
                                                         const int32_t* __restrict__ done) {
   const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   const int wv = w / F.max_chunks;
@@ -1064,8 +1086,8 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
   const int2 cd = chains[cbeg + wv];
   const int nb = __builtin_amdgcn_readfirstlane(cd.x);
   const int len = __builtin_amdgcn_readfirstlane(cd.y);
-  int u = __builtin_amdgcn_readfirstlane(cnodes[nb]);
-  const int t = F.tree_of[u];
+  ChainStep r = steps[nb];
+  const int t = F.tree_of[r.u];
   const int p = F.tree_prob[t];
   const int K = F.tree_nsites[t];
   const int chunk = w - wv * F.max_chunks;
@@ -1075,7 +1097,7 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
   const int k = chunk * 64 + lane;
   const bool on = k < K;
   OutNode n;
-  out_node_load<false>(F, u, t, p, K, n);
+  out_node_load<false>(F, r.u, t, p, K, n);
   OutSite x;
   double iL = 0.0;
   double4 au = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -1091,24 +1113,26 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
     iL = x.iL;
   }
   for (int j = 0;;) {
-    const int nxt = j + 1 < len ? __builtin_amdgcn_readfirstlane(cnodes[nb + j + 1]) : -1;
-    const int hv = nxt < 0 ? -1 : (n.ch[0] == nxt ? 0 : 1);
+    const bool more = j + 1 < len;
+    ChainStep rn = r;
+    if (more) rn = steps[nb + j + 1];
     double acc[2][5];
     zero_acc(acc);
     double rt = 0.0;
     if (on) {
       if (n.kind[0] == kLeaf) {
-        if (n.kind[1] == kLeaf) chain_site<kLeaf, kLeaf>(F, n, k, x, iL, hv, au, Eo, gu, acc, rt);
-        else chain_site<kLeaf, kInternal>(F, n, k, x, iL, hv, au, Eo, gu, acc, rt);
+        if (n.kind[1] == kLeaf) chain_site<kLeaf, kLeaf>(F, n, k, x, iL, r.hv, au, Eo, gu, acc, rt);
+        else chain_site<kLeaf, kInternal>(F, n, k, x, iL, r.hv, au, Eo, gu, acc, rt);
       } else {
-        if (n.kind[1] == kLeaf) chain_site<kInternal, kLeaf>(F, n, k, x, iL, hv, au, Eo, gu, acc, rt);
-        else chain_site<kInternal, kInternal>(F, n, k, x, iL, hv, au, Eo, gu, acc, rt);
+        if (n.kind[1] == kLeaf) chain_site<kInternal, kLeaf>(F, n, k, x, iL, r.hv, au, Eo, gu, acc, rt);
+        else chain_site<kInternal, kInternal>(F, n, k, x, iL, r.hv, au, Eo, gu, acc, rt);
       }
     }
-    out_write_partials<64>(F, u, n.ch, lane, chunk, chunk + 1, acc, rt);
-    if (++j >= len) break;
-    u = nxt;
-    out_node_load<false>(F, u, t, p, K, n);
+    out_write_partials<64>(F, r.u, n.ch, lane, chunk, chunk + 1, acc, rt);
+    if (!more) break;
+    ++j;
+    r = rn;
+    chain_step_node(F, r, n);
   }
 }
 
@@ -2348,7 +2372,7 @@ void run_outside_chains(const DevForest& F, const ChainPlan& cp, const int32_t* 
     if (cnt <= 0) continue;
     const int64_t waves = (int64_t)cnt * F.max_chunks;
     hipLaunchKernelGGL(k_outside_chain, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, F, cp.chains, beg, cnt,
-                       cp.nodes, done);
+                       cp.steps, done);
   }
 }
 

@@ -158,11 +158,20 @@ struct CherrySplit {
 // one round after the last chain hanging off it (its light internal
 // children's), round 0 holding chains with none (device arrays, host
 // offsets).
+// One node of a chain, as the outside chain kernel reads it: the node, its
+// children, which child continues the chain (-1: none), and the children's
+// row offsets.
+struct ChainStep {
+  int32_t u, ch0, ch1, hv;
+  int64_t ro0, ro1;
+};
+
 struct ChainPlan {
   int32_t h_split = -1;
   const int2* chains = nullptr;
   const int2* in_chains = nullptr;
   const int32_t* nodes = nullptr;
+  const ChainStep* steps = nullptr;  // [nodes] the record of each entry of nodes
   std::vector<int32_t> round_off, in_round_off;
 };
 
