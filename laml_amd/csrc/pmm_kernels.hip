@@ -1021,15 +1021,30 @@ __global__ void __launch_bounds__(256) k_outside_rows_ch(DevForest F, int beg, i
 // chunk per wavefront, so the results are the same bits.
 template <int K0, int K1>
 __device__ __forceinline__ void chain_site(const DevForest& F, const OutNode& n, int k, OutSite& x, double iL,
-                                           int hv, double4& au, int& Eo, int& gu, double (&acc)[2][5],
-                                           double& rt) {
-  const int kind[2] = {resolve_kind<K0>(n.kind[0]), resolve_kind<K1>(n.kind[1])};
-  const bool lf[2] = {kind[0] == kLeaf, kind[1] == kLeaf};
+                                           int hv, int raw0, int raw1, double q0, double q1, double4& au, int& Eo,
+                                           int& gu, double (&acc)[2][5], double& rt) {
+  const bool lf[2] = {K0 == kLeaf, K1 == kLeaf};
   x.au = au;
   x.Eo = Eo;
   x.gu = gu;
-  outside_load_children<K0, K1>(F, n, k, x);
-  outside_gather(F, x);
+  const int raw[2] = {raw0, raw1};
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    if (lf[i]) {
+      x.code[i] = raw[i];
+      leaf_beta(raw[i], n.phi, x.lab[i], x.b0[i], x.bA[i]);
+      x.E[i] = 0;
+    } else {
+      x.code[i] = 0;
+      x.lab[i] = raw[i];
+      const double2 bb = row_at(n.c[i].beta, k);
+      x.b0[i] = bb.x;
+      x.bA[i] = bb.y;
+      x.E[i] = row_at(n.c[i].bE, k);
+    }
+  }
+  x.q[0] = q0;
+  x.q[1] = q1;
   Eps ep[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) ep[i] = edge(x.lab[i], x.b0[i], x.bA[i], x.E[i], x.q[i], n.cf[i]);
@@ -1048,6 +1063,13 @@ __device__ __forceinline__ void chain_site(const DevForest& F, const OutNode& n,
                 row_at(n.caG[i], k) = (int16_t)gg;
               }
             });
+}
+
+// the prior of the state a child's raw row value (leaf code or az-partition
+// label) names at a site
+__device__ __forceinline__ double chain_q(const DevForest& F, bool leaf, int raw, int qo, double qs) {
+  const int lab = leaf ? code_label(raw) : raw;
+  return lab > 0 ? row_at(F.prior, (unsigned)(qo + lab)) : qs;
 }
 
 // The per-node values of one chain step, from its record: children, their
@@ -1112,27 +1134,57 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
     x.qs = row_at(n.qsum, k);
     iL = x.iL;
   }
+  // The children's codes / labels of the next step are loaded at the start
+  // of a step (its record is loaded two steps ahead) and their priors (the
+  // gathers that need those values) at its end, so a step's own loads are
+  // the inside rows only.
+  int raw0 = 0, raw1 = 0;
+  double q0 = 0.0, q1 = 0.0;
+  if (on) {
+    raw0 = row_at(n.c[0].code, k);
+    raw1 = row_at(n.c[1].code, k);
+    q0 = chain_q(F, n.kind[0] == kLeaf, raw0, x.qo, x.qs);
+    q1 = chain_q(F, n.kind[1] == kLeaf, raw1, x.qo, x.qs);
+  }
+  ChainStep rn = r;
+  if (len > 1) rn = steps[nb + 1];
   for (int j = 0;;) {
     const bool more = j + 1 < len;
-    ChainStep rn = r;
-    if (more) rn = steps[nb + j + 1];
+    ChainStep rnn = rn;
+    if (j + 2 < len) rnn = steps[nb + j + 2];
+    int rawn0 = 0, rawn1 = 0;
+    if (more && on) {
+      rawn0 = row_at((rn.ch0 < F.num_leaves ? F.codes : F.labels) + rn.ro0, k);
+      rawn1 = row_at((rn.ch1 < F.num_leaves ? F.codes : F.labels) + rn.ro1, k);
+    }
     double acc[2][5];
     zero_acc(acc);
     double rt = 0.0;
     if (on) {
       if (n.kind[0] == kLeaf) {
-        if (n.kind[1] == kLeaf) chain_site<kLeaf, kLeaf>(F, n, k, x, iL, r.hv, au, Eo, gu, acc, rt);
-        else chain_site<kLeaf, kInternal>(F, n, k, x, iL, r.hv, au, Eo, gu, acc, rt);
+        if (n.kind[1] == kLeaf)
+          chain_site<kLeaf, kLeaf>(F, n, k, x, iL, r.hv, raw0, raw1, q0, q1, au, Eo, gu, acc, rt);
+        else
+          chain_site<kLeaf, kInternal>(F, n, k, x, iL, r.hv, raw0, raw1, q0, q1, au, Eo, gu, acc, rt);
       } else {
-        if (n.kind[1] == kLeaf) chain_site<kInternal, kLeaf>(F, n, k, x, iL, r.hv, au, Eo, gu, acc, rt);
-        else chain_site<kInternal, kInternal>(F, n, k, x, iL, r.hv, au, Eo, gu, acc, rt);
+        if (n.kind[1] == kLeaf)
+          chain_site<kInternal, kLeaf>(F, n, k, x, iL, r.hv, raw0, raw1, q0, q1, au, Eo, gu, acc, rt);
+        else
+          chain_site<kInternal, kInternal>(F, n, k, x, iL, r.hv, raw0, raw1, q0, q1, au, Eo, gu, acc, rt);
       }
     }
     out_write_partials<64>(F, r.u, n.ch, lane, chunk, chunk + 1, acc, rt);
     if (!more) break;
     ++j;
     r = rn;
+    rn = rnn;
     chain_step_node(F, r, n);
+    if (on) {
+      raw0 = rawn0;
+      raw1 = rawn1;
+      q0 = chain_q(F, n.kind[0] == kLeaf, raw0, x.qo, x.qs);
+      q1 = chain_q(F, n.kind[1] == kLeaf, raw1, x.qo, x.qs);
+    }
   }
 }
 
