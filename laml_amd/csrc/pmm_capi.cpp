@@ -94,6 +94,7 @@ struct laml_forest {
   std::vector<laml::ChainPlan> chain_plans;
   std::vector<int32_t> h_heavy;  // heavy child of each node (-1: no internal child)
   std::vector<int32_t> h_child0, h_child1;
+  std::vector<int32_t> h_tree_of, h_tree_prob, h_tree_site_base;
   laml::TreeWalk TW{};
   // host copies for re-layout of the inside vectors
   std::vector<int32_t> h_parent;
@@ -346,6 +347,8 @@ int laml_forest_validate(const laml_forest_desc* d) {
   Derived D;
   return validate(d, D);
 }
+
+static int upload_node_recs(laml_forest* f, const std::vector<int64_t>& brow);
 
 int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out) {
   std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
@@ -609,6 +612,9 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   F.labels = labels;
   F.brow = F.row_off;
   f->h_parent.assign(d->parent, d->parent + N);
+  f->h_tree_of.assign(d->tree_of, d->tree_of + N);
+  f->h_tree_prob = tree_prob;
+  f->h_tree_site_base.assign(tree_site_base.begin(), tree_site_base.end());
   {
     // heavy child: the internal child with more leaves below it (children
     // precede their parents in level order)
@@ -633,6 +639,10 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
                     : (f->W == 64 && units >= 32768 ? LAML_CHAINS_OUTSIDE : 0);
   }
   f->h_row_off = row_off;
+  if ((rc = upload_node_recs(f, row_off))) {
+    delete f;
+    return rc;
+  }
   f->h_K = stride;
   f->h_desc_off = D.row_off;
   f->leaf_sites = D.leaf_rows;
@@ -784,6 +794,11 @@ int laml_forest_set_checkpoint(laml_forest* f, int32_t levels) {
     return rc;
   }
   LAML_HIP(hipMemcpy(db, brow.data(), sizeof(int64_t) * N, hipMemcpyHostToDevice));
+  if ((rc = upload_node_recs(f, brow))) {
+    (void)hipFree(nb);
+    (void)hipFree(ne);
+    return rc;
+  }
   if (f->beta_alloc) (void)hipFree(f->beta_alloc);
   if (f->bE_alloc) (void)hipFree(f->bE_alloc);
   f->beta_alloc = nb;
@@ -970,6 +985,34 @@ int laml_forest_ultra_status(laml_forest* f, int32_t* status_host) {
 static int check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(LAML_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+  return LAML_OK;
+}
+
+// uploads the per-node records of the inside row kernel for inside rows at
+// brow (rows of the stored inside vectors)
+static int upload_node_recs(laml_forest* f, const std::vector<int64_t>& brow) {
+  const int N = f->F.num_nodes, n = f->F.num_leaves;
+  std::vector<laml::NodeRec> rec(std::max(N - n, 1));
+  for (int v = n; v < N; ++v) {
+    laml::NodeRec& r = rec[v - n];
+    const int t = f->h_tree_of[v];
+    r.ch0 = f->h_child0[v];
+    r.ch1 = f->h_child1[v];
+    r.p = f->h_tree_prob[t];
+    r.K = f->tree_nsites[t];
+    r.sb = f->h_tree_site_base[t];
+    r.pad = 0;
+    r.ro = f->h_row_off[v];
+    r.bro = brow[v];
+    r.ro0 = f->h_row_off[r.ch0];
+    r.bro0 = brow[r.ch0];
+    r.ro1 = f->h_row_off[r.ch1];
+    r.bro1 = brow[r.ch1];
+  }
+  const laml::NodeRec* d = nullptr;
+  int rc = f->upload(&d, rec.data(), rec.size());
+  if (rc) return rc;
+  f->F.nrec = d;
   return LAML_OK;
 }
 

@@ -339,6 +339,31 @@ __device__ __forceinline__ int row_node(int beg, int cnt, const int32_t* __restr
   return nodes ? __builtin_amdgcn_readfirstlane(nodes[beg + wv]) : beg + wv;
 }
 
+// The inside-pass values of internal node v from its record (children that
+// are leaves or stored internal nodes).
+__device__ __forceinline__ void in_node_rec(const DevForest& F, const NodeRec& r, InNode& n) {
+  n.K = r.K;
+  const int ch[2] = {r.ch0, r.ch1};
+  const int64_t ro[2] = {r.ro0, r.ro1}, bro[2] = {r.bro0, r.bro1};
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    n.kind[i] = ch[i] < F.num_leaves ? kLeaf : kInternal;
+    n.cf[i] = F.coef[ch[i]];
+    n.c[i].code = (n.kind[i] == kLeaf ? F.codes : F.labels) + ro[i];
+    n.c[i].beta = F.beta + bro[i];
+    n.c[i].bE = F.bE + bro[i];
+    n.c[i].code2 = nullptr;
+    n.c[i].gc[0] = n.c[i].gc[1] = -1;
+  }
+  n.lab = F.labels + r.ro;
+  n.beta = F.beta + r.bro;
+  n.bE = F.bE + r.bro;
+  n.qoff = F.site_qoff + r.sb;
+  n.qsum = F.site_qsum + r.sb;
+  n.phi = F.phi[r.p];
+  n.stream = F.stream_rows != 0;
+}
+
 // Nodes whose children are leaves or stored internal nodes.
 __global__ void __launch_bounds__(256) k_inside_rows(DevForest F, int beg, int cnt, int cpw,
                                                       const int32_t* __restrict__ done,
@@ -346,16 +371,14 @@ __global__ void __launch_bounds__(256) k_inside_rows(DevForest F, int beg, int c
   const int v = row_node(beg, cnt, nodes);
   if (v < 0) return;
   const int lane = threadIdx.x & 63;
-  const int t = F.tree_of[v];
-  const int p = F.tree_prob[t];
-  const int K = F.tree_nsites[t];
-  const int nch = (K + 63) >> 6;
+  const NodeRec r = F.nrec[v - F.num_leaves];
+  const int nch = (r.K + 63) >> 6;
   const int c0 = (int)blockIdx.y * cpw;
   if (c0 >= nch) return;
-  if (done && done[p]) return;
+  if (done && done[r.p]) return;
   const int c1 = min(c0 + cpw, nch);
   InNode n;
-  in_node_load<false>(F, v, t, p, K, n);
+  in_node_rec(F, r, n);
   if (n.kind[0] == kLeaf) {
     if (n.kind[1] == kLeaf) inside_row_run<kLeaf, kLeaf>(F, n, c0, c1, lane);
     else inside_row_run<kLeaf, kInternal>(F, n, c0, c1, lane);

@@ -16,6 +16,17 @@ struct NuState {
   int32_t count;  // blocks arrived in the current evaluation
 };
 
+// Per internal node, what the inside row kernel reads before its row loads
+// (one scalar load instead of chains of dependent ones): children, problem,
+// sites and first site of the tree, and the row offsets of the node's labels
+// and inside row and of its children's.
+struct NodeRec {
+  int32_t ch0, ch1, p, K;
+  int32_t sb, pad;
+  int64_t ro, bro;
+  int64_t ro0, bro0, ro1, bro1;
+};
+
 // Device view of a forest; passed by value to every kernel.
 struct DevForest {
   int32_t num_nodes, num_leaves, num_trees, num_problems;
@@ -36,6 +47,7 @@ struct DevForest {
   const int32_t* tree_of;
   const int64_t* row_off;
   const int64_t* brow;  // [N] row of the node's inside vector (beta, bE)
+  const NodeRec* nrec;  // [N - num_leaves] by internal node
   const uint8_t* free_branch;
   // trees / problems
   const int32_t* tree_root;
