@@ -301,15 +301,38 @@ __device__ __forceinline__ void inside_site(const DevForest& F, const InNode& n,
   inside_finish(n, k, x);
 }
 
+// The inside-pass values of internal node v from its record (children that
+// are leaves or stored internal nodes).
+__device__ __forceinline__ void in_node_rec(const DevForest& F, const NodeRec& r, InNode& n) {
+  n.K = r.K;
+  const int ch[2] = {r.ch0, r.ch1};
+  const int64_t ro[2] = {r.ro0, r.ro1}, bro[2] = {r.bro0, r.bro1};
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    n.kind[i] = ch[i] < F.num_leaves ? kLeaf : kInternal;
+    n.cf[i] = F.coef[ch[i]];
+    n.c[i].code = (n.kind[i] == kLeaf ? F.codes : F.labels) + ro[i];
+    n.c[i].beta = F.beta + bro[i];
+    n.c[i].bE = F.bE + bro[i];
+    n.c[i].code2 = nullptr;
+    n.c[i].gc[0] = n.c[i].gc[1] = -1;
+  }
+  n.lab = F.labels + r.ro;
+  n.beta = F.beta + r.bro;
+  n.bE = F.bE + r.bro;
+  n.qoff = F.site_qoff + r.sb;
+  n.qsum = F.site_qsum + r.sb;
+  n.phi = F.phi[r.p];
+  n.stream = F.stream_rows != 0;
+}
+
 // segment form: one site k of node v
 __device__ __forceinline__ void inside_node(const DevForest& F, int v, int k, const int32_t* __restrict__ done) {
-  const int t = F.tree_of[v];
-  const int p = F.tree_prob[t];
-  const int K = F.tree_nsites[t];
-  if (k >= K) return;
-  if (done && done[p]) return;
+  const NodeRec r = F.nrec[v - F.num_leaves];
+  if (k >= r.K) return;
+  if (done && done[r.p]) return;
   InNode n;
-  in_node_load<false>(F, v, t, p, K, n);
+  in_node_rec(F, r, n);
   inside_site<kNoCherry, kNoCherry>(F, n, k);
 }
 
@@ -337,31 +360,6 @@ __device__ __forceinline__ int row_node(int beg, int cnt, const int32_t* __restr
   const int wv = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   if (wv >= cnt) return -1;
   return nodes ? __builtin_amdgcn_readfirstlane(nodes[beg + wv]) : beg + wv;
-}
-
-// The inside-pass values of internal node v from its record (children that
-// are leaves or stored internal nodes).
-__device__ __forceinline__ void in_node_rec(const DevForest& F, const NodeRec& r, InNode& n) {
-  n.K = r.K;
-  const int ch[2] = {r.ch0, r.ch1};
-  const int64_t ro[2] = {r.ro0, r.ro1}, bro[2] = {r.bro0, r.bro1};
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    n.kind[i] = ch[i] < F.num_leaves ? kLeaf : kInternal;
-    n.cf[i] = F.coef[ch[i]];
-    n.c[i].code = (n.kind[i] == kLeaf ? F.codes : F.labels) + ro[i];
-    n.c[i].beta = F.beta + bro[i];
-    n.c[i].bE = F.bE + bro[i];
-    n.c[i].code2 = nullptr;
-    n.c[i].gc[0] = n.c[i].gc[1] = -1;
-  }
-  n.lab = F.labels + r.ro;
-  n.beta = F.beta + r.bro;
-  n.bE = F.bE + r.bro;
-  n.qoff = F.site_qoff + r.sb;
-  n.qsum = F.site_qsum + r.sb;
-  n.phi = F.phi[r.p];
-  n.stream = F.stream_rows != 0;
 }
 
 // Nodes whose children are leaves or stored internal nodes.
@@ -667,6 +665,37 @@ __device__ __forceinline__ void out_node_load(const DevForest& F, int u, int t, 
   n.stream = F.stream_rows != 0;
 }
 
+// The outside-pass values of internal node u from its record (children that
+// are leaves or stored internal nodes).
+__device__ __forceinline__ void out_node_rec(const DevForest& F, const NodeRec& r, OutNode& n) {
+  n.K = r.K;
+  const int ch[2] = {r.ch0, r.ch1};
+  const int64_t ro[2] = {r.ro0, r.ro1}, bro[2] = {r.bro0, r.bro1};
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    n.ch[i] = ch[i];
+    n.kind[i] = ch[i] < F.num_leaves ? kLeaf : kInternal;
+    n.cf[i] = F.coef[ch[i]];
+    n.c[i].code = (n.kind[i] == kLeaf ? F.codes : F.labels) + ro[i];
+    n.c[i].beta = F.beta + bro[i];
+    n.c[i].bE = F.bE + bro[i];
+    n.c[i].code2 = nullptr;
+    n.c[i].gc[0] = n.c[i].gc[1] = -1;
+    n.ca[i] = F.alpha + ro[i];
+    n.caE[i] = F.aE + ro[i];
+    n.caG[i] = F.aG + ro[i];
+  }
+  n.a = F.alpha + r.ro;
+  n.aE = F.aE + r.ro;
+  n.aG = F.aG + r.ro;
+  n.iLm = F.iLm + r.sb;
+  n.LE = F.LE + r.sb;
+  n.qoff = F.site_qoff + r.sb;
+  n.qsum = F.site_qsum + r.sb;
+  n.phi = F.phi[r.p];
+  n.stream = F.stream_rows != 0;
+}
+
 // The posterior step of one node at one site.  In: the node's outside state
 // au 2^Eo with specific state gu, 1/L 2^-LE of the site, and for both
 // children their inside state (lab, b0, bA, E), leaf code, prior q, edge
@@ -940,13 +969,12 @@ template <int W>
 __device__ __forceinline__ void outside_node(const DevForest& F, int u, int chunk, int lane,
                                              const int32_t* __restrict__ done) {
   const int k = chunk * W + lane;
-  const int t = F.tree_of[u];
-  const int p = F.tree_prob[t];
-  const int K = F.tree_nsites[t];
+  const NodeRec r = F.nrec[u - F.num_leaves];
+  const int K = r.K;
   if (chunk * W >= K) return;
-  if (done && done[p]) return;
+  if (done && done[r.p]) return;
   OutNode n;
-  out_node_load<false>(F, u, t, p, K, n);
+  out_node_rec(F, r, n);
   double acc[2][5];
   zero_acc(acc);
   double rt = 0.0;
@@ -985,16 +1013,14 @@ __global__ void __launch_bounds__(256) k_outside_rows(DevForest F, int beg, int 
   const int u = row_node(beg, cnt, nodes);
   if (u < 0) return;
   const int lane = threadIdx.x & 63;
-  const int t = F.tree_of[u];
-  const int p = F.tree_prob[t];
-  const int K = F.tree_nsites[t];
-  const int nch = (K + 63) >> 6;
+  const NodeRec r = F.nrec[u - F.num_leaves];
+  const int nch = (r.K + 63) >> 6;
   const int c0 = (int)blockIdx.y * cpw;
   if (c0 >= nch) return;
-  if (done && done[p]) return;
+  if (done && done[r.p]) return;
   const int c1 = min(c0 + cpw, nch);
   OutNode n;
-  out_node_load<false>(F, u, t, p, K, n);
+  out_node_rec(F, r, n);
   if (n.kind[0] == kLeaf) {
     if (n.kind[1] == kLeaf) outside_row_run<kLeaf, kLeaf>(F, u, n, c0, c1, lane);
     else outside_row_run<kLeaf, kInternal>(F, u, n, c0, c1, lane);
@@ -1268,7 +1294,7 @@ __global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWal
         const int v = W == 64 ? __builtin_amdgcn_readfirstlane(beg + i) : beg + i;
         if (k < K) {
           InNode n;
-          in_node_load<false>(F, v, t, p, K, n);
+          in_node_rec(F, F.nrec[v - F.num_leaves], n);
           inside_site<kNoCherry, kNoCherry>(F, n, k);
         }
       }
@@ -1285,7 +1311,7 @@ __global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWal
       for (int i = seg; i < cnt; i += kSegs) {
         const int u = W == 64 ? __builtin_amdgcn_readfirstlane(beg + i) : beg + i;
         OutNode n;
-        out_node_load<false>(F, u, t, p, K, n);
+        out_node_rec(F, F.nrec[u - F.num_leaves], n);
         double acc[2][5];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
