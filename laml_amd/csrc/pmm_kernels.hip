@@ -348,9 +348,26 @@ __global__ void __launch_bounds__(256) k_inside(DevForest F, int beg, int cnt, c
 // loads and the child-kind tests scalar branches.
 template <int K0, int K1>
 __device__ __forceinline__ void inside_row_run(const DevForest& F, const InNode& n, int c0, int c1, int lane) {
+  // software-pipelined over the wave's chunks: the rows and prior gathers of
+  // chunk c + 1 are issued before chunk c's arithmetic and stores (they touch
+  // other sites, so the order of memory accesses is free)
+  InSite x;
+  int k = c0 * 64 + lane;
+  bool on = k < n.K;
+  if (on) {
+    inside_load<K0, K1>(F, n, k, x);
+    inside_gather(F, x);
+  }
   for (int c = c0; c < c1; ++c) {
-    const int k = c * 64 + lane;
-    if (k < n.K) inside_site<K0, K1>(F, n, k);
+    InSite y;
+    const int kn = k + 64;
+    const bool onn = c + 1 < c1 && kn < n.K;
+    if (onn) inside_load<K0, K1>(F, n, kn, y);
+    if (on) inside_finish(n, k, x);
+    if (onn) inside_gather(F, y);
+    x = y;
+    k = kn;
+    on = onn;
   }
 }
 
