@@ -611,10 +611,13 @@ def main():
     # launches of k_outside, the fused top-level kernel, k_root, k_counts,
     # k_rtilde), timed with events on the launch stream over resident inside
     # state
+    # chain regions: the levels below each pass's region top and below the split
     split = min(max(1, int(F.get_shapes()[2])), F.num_levels)
     flags = F.chains()
-    inside_b, outside_b = algorithmic_bytes(F, fused=F.fusion(), chain_split=split if flags & 1 else None,
-                                            in_chain_split=split if flags & 2 else None)
+    _, top_in, top_out = F.get_chains()
+    region = lambda t: split if t <= 0 else min(t, split)
+    inside_b, outside_b = algorithmic_bytes(F, fused=F.fusion(), chain_split=region(top_out) if flags & 1 else None,
+                                            in_chain_split=region(top_in) if flags & 2 else None)
     F.nll(d, nu, phi)
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
