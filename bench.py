@@ -521,6 +521,9 @@ def main():
     ap.add_argument("--batch", type=int, default=16, help="independent problems per step")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch plan of the batch forest (no autotuning before timing)")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="split the batch into this many forests, each stepped on its own HIP stream "
+                         "(laml_amd.streams.PipelinedBatch); 1: one forest")
     ap.add_argument("--save-plan", default=None,
                     help="write the batch forest's autotuned launch plan to this JSON file")
     ap.add_argument("--plan", default=None,
@@ -598,12 +601,40 @@ def main():
     log("single problem: %.3f ms/iter, %.1f EM it/s, nllh %.6f" % (el1 / args.steps * 1e3, single_per_s, single_nll))
     F1.close()
     del F1, d1, nu1, phi1
-    # the batch
-    F, d, nu, phi, nll, el, setup_s = run(probs, args.steps, args.warmup, tune=not args.no_autotune)
+    # the batch: one forest (also the roofline leg's forest), then, with
+    # --streams S > 1, the measured run on S forests stepped on S streams
+    F, d, nu, phi, nll, el, setup_s = run(probs, args.steps if args.streams <= 1 else 0,
+                                          args.warmup if args.streams <= 1 else 0, tune=not args.no_autotune)
+    execution = {"forests": 1, "streams": 1}
+    if args.streams > 1:
+        from laml_amd.streams import PipelinedBatch
+        t0 = time.perf_counter()
+        pb = PipelinedBatch([sim.problem_spec(t, q, c) for (t, q, c, _) in probs], parts=args.streams, device=0)
+        prm = pb.params([[np.full(t.n_nodes, 0.05)] for (t, _, _, _) in probs], [0.5] * B, [0.5] * B)
+        setup_s = time.perf_counter() - t0
+        plans = pb.autotune(prm) if not args.no_autotune else None
+        # warm-up steps, then the timed steps from where they end (as for
+        # the single problem)
+        if args.warmup > 0:
+            pb.em(prm, args.warmup, 0.0)
+        pb.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = pb.em(prm, args.steps, 0.0)
+        pb.synchronize()
+        el = time.perf_counter() - t0
+        j, i0 = pb.locate(0)
+        nll = res[j][0][i0:i0 + 1]
+        execution = {"forests": len(pb.forests), "streams": len(pb.streams),
+                     "problems_per_forest": [len(g) for g in pb.groups],
+                     "plans": [{k: v for k, v in p.describe().items() if "cpw" not in k} for p in plans] if plans
+                     else "default"}
+        pb.close()
     em_per_s = B * args.steps / el
     final_nll = float(nll[0].item())
-    log("batch of %d: %.3f ms/iter, %.1f aggregate EM it/s (x%.1f the single-problem rate); setup %.2f s" % (
-        B, el / args.steps * 1e3, em_per_s, em_per_s / single_per_s, setup_s))
+    log("batch of %d: %.3f ms/iter, %.1f aggregate EM it/s (x%.1f the single-problem rate); setup %.2f s; %s" % (
+        B, el / args.steps * 1e3, em_per_s, em_per_s / single_per_s, setup_s,
+        "one forest" if args.streams <= 1 else "%d forests on %d streams" % (execution["forests"], execution["streams"])))
     if abs(final_nll - single_nll) > 1e-9 * abs(single_nll):
         raise SystemExit("batched result differs from the single-problem result: %r vs %r" % (final_nll, single_nll))
 
@@ -660,7 +691,7 @@ def main():
                                "tree x %d sites, M in [%d,%d], Dirichlet(%g), nu=%g, phi=%g, seeds %d+1000*b"
                                % (B, n, K, mlo, mhi, alpha, nu_true, phi_true, seed),
                    "batch": B, "leaves": n, "sites": K, "nodes": F.num_nodes, "levels": F.num_levels,
-                   "launch_plan": plan_desc or "default"},
+                   "launch_plan": plan_desc or "default", "execution": execution},
         "node_site_evals_per_s": round(node_site * em_per_s, 1),
         "final_nllh_problem0": final_nll,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

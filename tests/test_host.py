@@ -236,3 +236,21 @@ def test_chain_byte_accounting():
         assert ins == base[0] and out == base[1] - saved
         ins, out = bench.algorithmic_bytes(F, in_chain_split=split)
         assert out == base[1] and ins == base[0] - saved // 76 * 22
+
+
+def test_pipelined_batch_split():
+    """Contiguous, non-empty, balanced groups of problems for the streams."""
+    from laml_amd.streams import split_balanced
+
+    probs = [sim.make_problem(n, K, 3, 10, 1.0, 0.1, 0.1, 70 + i)
+             for i, (n, K) in enumerate([(300, 40), (50, 40), (50, 40), (600, 40), (40, 10), (200, 80)])]
+    specs = [sim.problem_spec(t, q, c) for (t, q, c, _) in probs]
+    for parts in (1, 2, 3, 6, 9):
+        g = split_balanced(specs, parts)
+        assert len(g) == min(parts, len(specs))
+        assert [i for grp in g for i in grp] == list(range(len(specs)))
+        assert all(grp for grp in g)
+    w = [sum(t.n_nodes for t in p.trees) * p.sites.K for p in specs]
+    g = split_balanced(specs, 2)
+    halves = [sum(w[i] for i in grp) for grp in g]
+    assert max(halves) <= sum(w) - min(w)
