@@ -122,6 +122,18 @@ def outside_traffic_from_profile(args):
     return (int(float(m.group(1)) * 1e9) if m else None), (float(v.group(1)) if v else None)
 
 
+def profile_plan():
+    """The launch plan the committed profile summary was measured with (the
+    roofline leg applies it, so that the profile's counters describe the
+    launches it times); None if absent."""
+    import re
+    path = os.path.join(ROOT, PROFILE)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"applies it to its roofline leg\): `(\{.*?\})`", open(path).read())
+    return json.loads(m.group(1)) if m else None
+
+
 def cpu_baseline_python(tree, Q, codes, target_s=4.0):
     """Time the scalar Python oracle (a restatement of EM_solver with the
     reference's own per-(node, site) Python loops) on a bounded sample: the
@@ -534,7 +546,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the PMM path on a single MI355X "
-              "(see DESIGN.md for how it would shard); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+              "(multi-GPU sharding by tree or by site is designed, not built); --gpus %d is not supported"
+              % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     if args.workload in SCORING:
@@ -603,8 +616,10 @@ def main():
     del F1, d1, nu1, phi1
     # the batch: one forest (also the roofline leg's forest), then, with
     # --streams S > 1, the measured run on S forests stepped on S streams
+    roof_plan = profile_plan() if (B == 16 and args.workload == "em10k" and not args.plan) else None
     F, d, nu, phi, nll, el, setup_s = run(probs, args.steps if args.streams <= 1 else 0,
-                                          args.warmup if args.streams <= 1 else 0, tune=not args.no_autotune)
+                                          args.warmup if args.streams <= 1 else 0,
+                                          tune=not args.no_autotune and (args.streams <= 1 or roof_plan is None))
     execution = {"forests": 1, "streams": 1}
     if args.streams > 1:
         from laml_amd.streams import PipelinedBatch
@@ -637,6 +652,14 @@ def main():
         "one forest" if args.streams <= 1 else "%d forests on %d streams" % (execution["forests"], execution["streams"])))
     if abs(final_nll - single_nll) > 1e-9 * abs(single_nll):
         raise SystemExit("batched result differs from the single-problem result: %r vs %r" % (final_nll, single_nll))
+
+    # the roofline leg runs the plan of the committed profile of this workload
+    # when there is one (its traffic and issue counters then describe these
+    # launches)
+    if roof_plan is not None:
+        from laml_amd.autotune import plan_from_dict
+        plan_from_dict(roof_plan, F).apply(F)
+        log("roofline leg: launch plan of %s" % PROFILE)
 
     # roofline of the dominant kernel family: the outside pass (all level
     # launches of k_outside, the fused top-level kernel, k_root, k_counts,
@@ -691,13 +714,15 @@ def main():
                                "tree x %d sites, M in [%d,%d], Dirichlet(%g), nu=%g, phi=%g, seeds %d+1000*b"
                                % (B, n, K, mlo, mhi, alpha, nu_true, phi_true, seed),
                    "batch": B, "leaves": n, "sites": K, "nodes": F.num_nodes, "levels": F.num_levels,
-                   "launch_plan": plan_desc or "default", "execution": execution},
+                   "launch_plan": (plan_desc or "default") if args.streams <= 1 else "per forest, see execution",
+                   "execution": execution},
         "node_site_evals_per_s": round(node_site * em_per_s, 1),
         "final_nllh_problem0": final_nll,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": PROFILE if traffic is not None else None,
                      "valu_issue_frac": valu,
+                     "launch_plan": "as in " + PROFILE if roof_plan is not None else "as the batch",
                      "kernel": "outside pass (laml_outside: k_tree_walk top levels + root in the hybrid plan, "
                                "k_outside_chain rounds below the split when the plan has chains, else "
                                "k_outside_rows levels, k_counts, k_rtilde)",
