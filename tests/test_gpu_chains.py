@@ -237,3 +237,28 @@ def test_chain_eligibility():
     Fa.set_fusion(False)
     assert Fa.chains()
     _close(Fa, Fb)
+
+
+def test_default_plan_with_chains_matches_c_oracle_em():
+    """A forest big enough that outside chains are on by default (7000
+    leaves x 300 sites) runs the same EM as the C oracle (oracle/pmm_ref.c):
+    five whole iterations from the benchmark's start."""
+    from oracle import pmm_ref
+
+    t, q, c, _ = sim.make_problem(7000, 300, 5, 30, 1.0, 0.1, 0.1, 2024)
+    F = Forest([sim.problem_spec(t, q, c)], device=torch.cuda.current_device())
+    assert F.chains() == 1
+    d = torch.tensor(F.lengths_to_level([np.full(t.n_nodes, 0.05)]), dtype=torch.float64, device=F.device)
+    nu = torch.full((1,), 0.5, dtype=torch.float64, device=F.device)
+    phi = torch.full((1,), 0.5, dtype=torch.float64, device=F.device)
+    nll, _ = F.em(d, nu, phi, 5, 0.0)
+    got_nll = float(nll.cpu().numpy()[0])
+    got_d = F.level_to_tree_lengths(d.cpu().numpy())[0]
+    p = pmm_ref.RefProblem(t.c0, t.c1, t.parent, c, q, None, 1.0 / (len(q) * c.shape[0]))
+    rn, _, rd, rnu, rphi = p.em(np.full(t.n_nodes, 0.05), 0.5, 0.5, max_iter=5, tol=0.0, fixed_iters=5)
+    print("device", got_nll, "oracle", rn, flush=True)
+    assert math.isclose(got_nll, rn, rel_tol=1e-9)
+    assert abs(nu.item() - rnu) <= 1e-6 * max(1.0, rnu)
+    assert abs(phi.item() - rphi) <= 1e-6 * max(1.0, rphi)
+    np.testing.assert_allclose(got_d, rd, rtol=1e-6, atol=1e-9)
+    F.close()
