@@ -1049,6 +1049,10 @@ static int chains_now(const laml_forest* f) {
 static const laml::ChainPlan* chain_plan(laml_forest* f, int hs) {
   for (const laml::ChainPlan& cp : f->chain_plans)
     if (cp.h_split == hs) return &cp;
+  // a new plan is uploaded with synchronous copies: not while another
+  // thread captures its EM step (the passes only look plans up: each entry
+  // point builds them before any capture)
+  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
   const int N = f->F.num_nodes;
   const int lo = f->level_off[1], B = f->level_off[hs];
   std::vector<int32_t> chain_of(N, -1), round_of(N, 0);

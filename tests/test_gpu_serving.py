@@ -46,3 +46,24 @@ def test_request_over_budget_rejected():
     with PMMServer(slots=1, hbm_budget=r.bytes - 1) as srv:
         with pytest.raises(ValueError):
             srv.submit(r)
+
+
+def test_concurrent_requests_with_chain_plans():
+    """Requests big enough for outside chains by default, served four at a
+    time: each worker builds its forest's chain plan while others capture
+    their EM steps; results equal solving alone, bit for bit."""
+    reqs = []
+    for i, (n, K) in enumerate([(7000, 300), (9000, 256), (12000, 192), (6600, 320)]):
+        tree, Q, codes, _ = sim.make_problem(n, K, 5, 40, 1.0, 0.1, 0.1, 4100 + i)
+        spec = sim.problem_spec(tree, Q, codes)
+        reqs.append(SolveRequest(spec, [np.full(tree.n_nodes, 0.05)], 0.5, 0.5, max_iter=20, tol=1e-8, rid=i))
+    alone = {r.rid: solve_alone(r) for r in reqs}
+    with PMMServer(slots=4) as srv:
+        futs = [srv.submit(r) for r in reqs]
+        res = [f.result(timeout=300) for f in futs]
+    for r in res:
+        a = alone[r.rid]
+        print("request", r.rid, r.nll, a[0], flush=True)
+        assert r.nll == a[0] and r.nu == a[2] and r.phi == a[3] and r.iters == a[4]
+        for x, y in zip(r.lengths, a[1]):
+            np.testing.assert_array_equal(x, y)
