@@ -722,6 +722,11 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": PROFILE if traffic is not None else None,
                      "valu_issue_frac": valu,
+                     "traffic_gbs": round(traffic / (out_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "note": "achieved counts the compulsory bytes of the algorithm the plan runs (outside chains "
+                             "keep heavy children's outside states in registers, about half the bytes of level "
+                             "launches); the pass is bound by the latency of each chain step's dependent loads, "
+                             "not by HBM bandwidth (traffic_gbs) or VALU issue (valu_issue_frac)",
                      "launch_plan": "as in " + PROFILE if roof_plan is not None else "as the batch",
                      "kernel": "outside pass (laml_outside: k_tree_walk top levels + root in the hybrid plan, "
                                "k_outside_chain rounds below the split when the plan has chains, else "
