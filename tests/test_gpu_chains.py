@@ -262,3 +262,29 @@ def test_default_plan_with_chains_matches_c_oracle_em():
     assert abs(phi.item() - rphi) <= 1e-6 * max(1.0, rphi)
     np.testing.assert_allclose(got_d, rd, rtol=1e-6, atol=1e-9)
     F.close()
+
+
+def test_ultrametric_em_with_chains_equals_levels():
+    """The ultrametric M-step on top of chain passes: the same EM trace and
+    parameters, bit for bit, as on level passes."""
+    from laml_amd.solvers import EM_solver
+
+    t, q, c, _ = sim.make_problem(150, 80, 3, 15, 1.0, 0.1, 0.1, 1900)
+    msa = sim.codes_to_charmtrx(t, c)
+    out = []
+    for on in (True, False):
+        s = EM_solver([t.newick()], {"charMtrx": msa}, {"Q": q}, {"nu": 0.1, "phi": 0.05})
+        F = s._forest_for()
+        F.set_pass_mode(0)
+        H = F.num_levels
+        F.set_shapes(np.ones(H, dtype=np.int32), np.ones(H, dtype=np.int32), H)
+        F.set_chains(on, 0, 0)
+        assert bool(F.chains()) == on
+        d, nu_t, phi_t = s._device_params(F)
+        assert s._enable_ultra(F, d.cpu().numpy())
+        trace = torch.zeros(6, dtype=torch.float64, device=F.device)
+        F.em(d, nu_t, phi_t, 5, 0.0, trace=trace)
+        out.append((trace.cpu().numpy().copy(), d.cpu().numpy().copy(), nu_t.cpu().numpy().copy(),
+                    phi_t.cpu().numpy().copy()))
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
