@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <deque>
 #include <tuple>
 #include <limits>
 #include <mutex>
@@ -91,7 +92,7 @@ struct laml_forest {
   // 0: up to the split
   int chain_top_in = 5, chain_top_out = 10;
   bool alpha_complete = true;
-  std::vector<laml::ChainPlan> chain_plans;
+  std::deque<laml::ChainPlan> chain_plans;  // a deque: plans stay put when more are added
   std::vector<int32_t> h_heavy;  // heavy child of each node (-1: no internal child)
   std::vector<int32_t> h_child0, h_child1;
   std::vector<int32_t> h_tree_of, h_tree_prob, h_tree_site_base;
