@@ -288,3 +288,41 @@ def test_ultrametric_em_with_chains_equals_levels():
                     phi_t.cpu().numpy().copy()))
     for a, b in zip(*out):
         np.testing.assert_array_equal(a, b)
+
+
+def test_chains_multi_tree_problem_and_heavy_dropout():
+    """Problems of several trees (one shared nu, phi) and data with heavy
+    silencing and dropout (many '?' labels): chain passes equal level
+    passes bit for bit, E-step and EM."""
+    from laml_amd.forest import ProblemSpec
+
+    rng = np.random.Generator(np.random.PCG64(1950))
+    q = sim.random_priors(96, 3, 20, 1.0, rng)
+    trees = [sim.yule_tree(n, rng) for n in (400, 250, 90)]
+    codes = [sim.simulate_codes(t, q, 0.9, 0.6, rng) for t in trees]
+    parts = [sim.problem_spec(t, q, c) for t, c in zip(trees, codes)]
+    multi = ProblemSpec([p.trees[0] for p in parts], parts[0].sites, sum(p.nobs for p in parts),
+                        min(p.phimin for p in parts))
+    t4, q4, c4, _ = sim.make_problem(333, 70, 3, 15, 1.0, 1.2, 0.5, 1951)
+    specs = [multi, sim.problem_spec(t4, q4, c4)]
+    lengths = [t.length for t in trees] + [t4.length]
+    dev = torch.cuda.current_device()
+    res = []
+    for on in (True, False):
+        F = Forest(specs, device=dev)
+        F.set_pass_mode(2)
+        H = F.num_levels
+        F.set_shapes(np.ones(H, dtype=np.int32), np.ones(H, dtype=np.int32), min(7, H))
+        F.set_chains(on, 0, 0)
+        assert bool(F.chains()) == on
+        d = torch.tensor(F.lengths_to_level(lengths), dtype=torch.float64, device=F.device)
+        nu = torch.tensor([0.9, 1.2], dtype=torch.float64, device=F.device)
+        phi = torch.tensor([0.6, 0.5], dtype=torch.float64, device=F.device)
+        counts, rt, nll = F.estep(d, nu, phi)
+        e = (counts.cpu().numpy().copy(), rt.cpu().numpy().copy(), nll.cpu().numpy().copy())
+        nll2, _ = F.em(d, nu, phi, 10, 0.0)
+        res.append(e + (nll2.cpu().numpy().copy(), d.cpu().numpy().copy(), nu.cpu().numpy().copy(),
+                        phi.cpu().numpy().copy()))
+        F.close()
+    for a, b in zip(*res):
+        np.testing.assert_array_equal(a, b)
