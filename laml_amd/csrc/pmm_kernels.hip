@@ -97,6 +97,24 @@ __device__ __forceinline__ const T& row_at(const T* p, unsigned k) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + k * (unsigned)sizeof(T));
 }
 
+// A load of wave-uniform data that no kernel writes while it runs (launch
+// plans, branch coefficients), through the constant address space: the
+// compiler may then issue it as a scalar load into SGPRs even after the
+// kernel's own stores, where it must otherwise assume the store may have
+// changed it and load it into VGPRs of every lane.
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "loaded as 32-bit words");
+  typedef const __attribute__((address_space(4))) uint32_t* WordPtr;
+  const WordPtr q = (WordPtr)(uintptr_t)p;
+  uint32_t w[sizeof(T) / 4];
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(T) / 4; ++i) w[i] = q[i];
+  T out;
+  __builtin_memcpy(&out, w, sizeof(T));
+  return out;
+}
+
 // Kinds of child a pass kernel meets.  A *cherry* is an internal node whose
 // children are both leaves; when the pass fuses cherries (F.cherry_end >
 // F.num_leaves: nodes [num_leaves, cherry_end) are exactly the cherries, the
@@ -1148,7 +1166,7 @@ __device__ __forceinline__ void chain_step_node(const DevForest& F, const ChainS
   for (int i = 0; i < 2; ++i) {
     n.ch[i] = ch[i];
     n.kind[i] = ch[i] < F.num_leaves ? kLeaf : kInternal;
-    n.cf[i] = F.coef[ch[i]];
+    n.cf[i] = uniform_load(F.coef + ch[i]);
     n.c[i].code = (n.kind[i] == kLeaf ? F.codes : F.labels) + ro[i];
     n.c[i].beta = F.beta + ro[i];
     n.c[i].bE = F.bE + ro[i];
@@ -1217,7 +1235,7 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
   for (int j = 0;;) {
     const bool more = j + 1 < len;
     ChainStep rnn = rn;
-    if (j + 2 < len) rnn = steps[nb + j + 2];
+    if (j + 2 < len) rnn = uniform_load(steps + nb + j + 2);
     int rawn0 = 0, rawn1 = 0;
     if (more && on) {
       rawn0 = row_at((rn.ch0 < F.num_leaves ? F.codes : F.labels) + rn.ro0, k);
