@@ -97,11 +97,12 @@ __device__ __forceinline__ const T& row_at(const T* p, unsigned k) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + k * (unsigned)sizeof(T));
 }
 
-// A load of wave-uniform data that no kernel writes while it runs (launch
-// plans, branch coefficients), through the constant address space: the
-// compiler may then issue it as a scalar load into SGPRs even after the
-// kernel's own stores, where it must otherwise assume the store may have
-// changed it and load it into VGPRs of every lane.
+// A load of data that no kernel writes while it runs (launch plans, node
+// records, branch coefficients), through the constant address space: at a
+// wave-uniform address the compiler may then issue it as a scalar load into
+// SGPRs even after the kernel's own stores, where it must otherwise assume
+// the store may have changed it and load it into VGPRs of every lane (other
+// addresses stay vector loads).
 template <typename T>
 __device__ __forceinline__ T uniform_load(const T* p) {
   static_assert(sizeof(T) % 4 == 0, "loaded as 32-bit words");
@@ -328,7 +329,7 @@ __device__ __forceinline__ void in_node_rec(const DevForest& F, const NodeRec& r
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     n.kind[i] = ch[i] < F.num_leaves ? kLeaf : kInternal;
-    n.cf[i] = F.coef[ch[i]];
+    n.cf[i] = uniform_load(F.coef + ch[i]);
     n.c[i].code = (n.kind[i] == kLeaf ? F.codes : F.labels) + ro[i];
     n.c[i].beta = F.beta + bro[i];
     n.c[i].bE = F.bE + bro[i];
@@ -710,7 +711,7 @@ __device__ __forceinline__ void out_node_rec(const DevForest& F, const NodeRec& 
   for (int i = 0; i < 2; ++i) {
     n.ch[i] = ch[i];
     n.kind[i] = ch[i] < F.num_leaves ? kLeaf : kInternal;
-    n.cf[i] = F.coef[ch[i]];
+    n.cf[i] = uniform_load(F.coef + ch[i]);
     n.c[i].code = (n.kind[i] == kLeaf ? F.codes : F.labels) + ro[i];
     n.c[i].beta = F.beta + bro[i];
     n.c[i].bE = F.bE + bro[i];
@@ -1324,12 +1325,12 @@ __global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWal
   const int hb = max(1, h_lo);
   if (phases & 1) {
     for (int h = hb; h < H; ++h) {
-      const int beg = TW.lvl_beg[lb + h], cnt = TW.lvl_cnt[lb + h];
+      const int beg = uniform_load(TW.lvl_beg + lb + h), cnt = uniform_load(TW.lvl_cnt + lb + h);
       for (int i = seg; i < cnt; i += kSegs) {
         const int v = W == 64 ? __builtin_amdgcn_readfirstlane(beg + i) : beg + i;
         if (k < K) {
           InNode n;
-          in_node_rec(F, F.nrec[v - F.num_leaves], n);
+          in_node_rec(F, uniform_load(F.nrec + (v - F.num_leaves)), n);
           inside_site<kNoCherry, kNoCherry>(F, n, k);
         }
       }
@@ -1342,11 +1343,11 @@ __global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWal
   }
   if (phases & 4) {
     for (int h = H - 1; h >= hb; --h) {
-      const int beg = TW.lvl_beg[lb + h], cnt = TW.lvl_cnt[lb + h];
+      const int beg = uniform_load(TW.lvl_beg + lb + h), cnt = uniform_load(TW.lvl_cnt + lb + h);
       for (int i = seg; i < cnt; i += kSegs) {
         const int u = W == 64 ? __builtin_amdgcn_readfirstlane(beg + i) : beg + i;
         OutNode n;
-        out_node_rec(F, F.nrec[u - F.num_leaves], n);
+        out_node_rec(F, uniform_load(F.nrec + (u - F.num_leaves)), n);
         double acc[2][5];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
