@@ -1362,17 +1362,19 @@ __global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWal
   }
 }
 
-// Per-node counts: fixed-order sum of the per-chunk partials.
-__global__ void k_counts(DevForest F, double* __restrict__ counts) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= F.num_nodes) return;
+// Per-node counts: fixed-order sum of the per-chunk partials, one thread per
+// (node, count), so that a wavefront's loads of one chunk slot cover a few
+// whole partial records instead of one value per 200-byte record.
+__global__ void __launch_bounds__(256) k_counts(DevForest F, double* __restrict__ counts) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)F.num_nodes * 5) return;
+  const int v = (int)(idx / 5);
+  const int j = (int)(idx - (int64_t)v * 5);
   const int C = (F.tree_nsites[F.tree_of[v]] + F.chunk_w - 1) / F.chunk_w;
-  double s[5] = {0, 0, 0, 0, 0};
-  for (int c = 0; c < C; ++c)
-#pragma unroll
-    for (int j = 0; j < 5; ++j) s[j] += F.cpart[((int64_t)v * F.max_chunks + c) * 5 + j];
-#pragma unroll
-  for (int j = 0; j < 5; ++j) counts[(int64_t)v * 5 + j] = s[j];
+  const double* src = F.cpart + (int64_t)v * F.max_chunks * 5 + j;
+  double s = 0.0;
+  for (int c = 0; c < C; ++c) s += src[c * 5];
+  counts[idx] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -1437,9 +1439,19 @@ __device__ __forceinline__ double problem_rtilde(const DevForest& F, int p, doub
   double s = 0.0;
   const int b = F.prob_int_off[p], e = F.prob_node_off[p + 1];
   const int C = (F.tree_nsites[F.prob_tree_off[p]] + F.chunk_w - 1) / F.chunk_w;
+  constexpr int kC = 8;  // chunk partials of a node loaded together
   for (int i = b + threadIdx.x; i < e; i += kRedThreads) {
-    const int64_t v = F.prob_nodes[i];
-    for (int c = 0; c < C; ++c) s += F.rt_part[v * F.max_chunks + c];
+    const double* src = F.rt_part + (int64_t)F.prob_nodes[i] * F.max_chunks;
+    if (C <= kC) {
+      double x[kC];
+#pragma unroll
+      for (int c = 0; c < kC; ++c) x[c] = c < C ? src[c] : 0.0;
+#pragma unroll
+      for (int c = 0; c < kC; ++c)
+        if (c < C) s += x[c];
+    } else {
+      for (int c = 0; c < C; ++c) s += src[c];
+    }
   }
   return block_sum<kRedThreads>(s, sh);
 }
@@ -2599,7 +2611,8 @@ void run_node_post(const DevForest& F, int W, int64_t leaf_rows, double* p0, dou
 }
 
 void run_counts(const DevForest& F, double* counts, hipStream_t s) {
-  hipLaunchKernelGGL(k_counts, dim3((F.num_nodes + 255) / 256), dim3(256), 0, s, F, counts);
+  hipLaunchKernelGGL(k_counts, dim3((unsigned)(((int64_t)F.num_nodes * 5 + 255) / 256)), dim3(256), 0, s, F,
+                     counts);
 }
 
 // clears the EM bookkeeping of every problem (a kernel rather than a memset,
