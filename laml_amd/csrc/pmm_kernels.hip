@@ -341,7 +341,7 @@ __device__ __forceinline__ void in_node_rec(const DevForest& F, const NodeRec& r
   n.bE = F.bE + r.bro;
   n.qoff = F.site_qoff + r.sb;
   n.qsum = F.site_qsum + r.sb;
-  n.phi = F.phi[r.p];
+  n.phi = uniform_load(F.phi + r.p);
   n.stream = F.stream_rows != 0;
 }
 
@@ -538,10 +538,18 @@ __global__ void __launch_bounds__(256) k_inside_chain(DevForest F, const int2* _
     qs = F.site_qsum[sb + k];
   }
   InCarry c{0, 0, 0.0, 0.0};
+  // each step's node record is loaded one step ahead
+  NodeRec r = uniform_load(F.nrec + (v - F.num_leaves));
   for (int j = len - 1;; --j) {
+    NodeRec rn = r;
+    int vn = v;
+    if (j > 0) {
+      vn = uniform_load(cnodes + nb + j - 1);
+      rn = uniform_load(F.nrec + (vn - F.num_leaves));
+    }
     InNode n;
-    in_node_load<false>(F, v, t, p, K, n);
-    const int hv = prev < 0 ? -1 : (F.child0[v] == prev ? 0 : 1);
+    in_node_rec(F, r, n);
+    const int hv = prev < 0 ? -1 : (r.ch0 == prev ? 0 : 1);
     if (on) {
       if (hv < 0) {
         chain_in_site<-1, kLeaf>(F, n, k, qo, qs, c);
@@ -555,7 +563,8 @@ __global__ void __launch_bounds__(256) k_inside_chain(DevForest F, const int2* _
     }
     if (j == 0) break;
     prev = v;
-    v = __builtin_amdgcn_readfirstlane(cnodes[nb + j - 1]);
+    v = vn;
+    r = rn;
   }
 }
 
@@ -728,7 +737,7 @@ __device__ __forceinline__ void out_node_rec(const DevForest& F, const NodeRec& 
   n.LE = F.LE + r.sb;
   n.qoff = F.site_qoff + r.sb;
   n.qsum = F.site_qsum + r.sb;
-  n.phi = F.phi[r.p];
+  n.phi = uniform_load(F.phi + r.p);
   n.stream = F.stream_rows != 0;
 }
 
