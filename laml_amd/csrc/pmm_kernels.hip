@@ -1607,11 +1607,18 @@ __global__ void __launch_bounds__(kNuThreads) k_nu_eval(DevForest F, const doubl
     if (F.prob_blk_off[mid] <= blk) lo_p = mid; else hi_p = mid;
   }
   const int p = lo_p;
-  if (done && done[p]) return;
   NuState* st = F.nu_state + p;
+  if (iter > 0) {
+    // set by the first evaluation for problems that are done or keep nu
+    // fixed, and by the evaluation that converges: the remaining launches of
+    // a captured M-step cost one load
+    if (st->fin) return;
+  } else if (done && done[p]) {
+    if (blk == F.prob_blk_off[p] && threadIdx.x == 0) st->fin = 1;
+    return;
+  }
   const uint32_t flags = F.prob_flags[p];
   const bool opt_nu = (flags & 2u) && F.prob_nfree[p] > 0;
-  if (iter > 0 && (!opt_nu || st->fin)) return;
   const int part = blk - F.prob_blk_off[p];
   const int nparts = F.prob_blk_off[p + 1] - F.prob_blk_off[p];
   const int b = F.prob_node_off[p], e = F.prob_node_off[p + 1];
