@@ -1443,25 +1443,29 @@ __global__ void __launch_bounds__(256) k_llh(DevForest F, double* __restrict__ n
 
 constexpr int kRedThreads = 1024;
 
+// adds node v's per-chunk dropout sums, chunks 0 .. C-1 in order, to s; up
+// to 8 chunks are loaded together before the additions
+__device__ __forceinline__ void add_node_rt(const DevForest& F, int64_t v, int C, double& s) {
+  constexpr int kC = 8;
+  const double* src = F.rt_part + v * F.max_chunks;
+  if (C <= kC) {
+    double x[kC];
+#pragma unroll
+    for (int c = 0; c < kC; ++c) x[c] = c < C ? src[c] : 0.0;
+#pragma unroll
+    for (int c = 0; c < kC; ++c)
+      if (c < C) s += x[c];
+  } else {
+    for (int c = 0; c < C; ++c) s += src[c];
+  }
+}
+
 // sum over the problem's internal nodes of the per-node dropout sums
 __device__ __forceinline__ double problem_rtilde(const DevForest& F, int p, double* sh) {
   double s = 0.0;
   const int b = F.prob_int_off[p], e = F.prob_node_off[p + 1];
   const int C = (F.tree_nsites[F.prob_tree_off[p]] + F.chunk_w - 1) / F.chunk_w;
-  constexpr int kC = 8;  // chunk partials of a node loaded together
-  for (int i = b + threadIdx.x; i < e; i += kRedThreads) {
-    const double* src = F.rt_part + (int64_t)F.prob_nodes[i] * F.max_chunks;
-    if (C <= kC) {
-      double x[kC];
-#pragma unroll
-      for (int c = 0; c < kC; ++c) x[c] = c < C ? src[c] : 0.0;
-#pragma unroll
-      for (int c = 0; c < kC; ++c)
-        if (c < C) s += x[c];
-    } else {
-      for (int c = 0; c < C; ++c) s += src[c];
-    }
-  }
+  for (int i = b + threadIdx.x; i < e; i += kRedThreads) add_node_rt(F, F.prob_nodes[i], C, s);
   return block_sum<kRedThreads>(s, sh);
 }
 
@@ -1640,10 +1644,8 @@ __global__ void __launch_bounds__(kNuThreads) k_nu_eval(DevForest F, const doubl
   }
   if (iter == 0 && (flags & 1u) && !rtilde_in) {
     const int C = (F.tree_nsites[F.prob_tree_off[p]] + F.chunk_w - 1) / F.chunk_w;
-    for (int i = max(pb, F.prob_int_off[p]) + threadIdx.x; i < pe; i += kNuThreads) {
-      const int64_t v = F.prob_nodes[i];
-      for (int c = 0; c < C; ++c) rt += F.rt_part[v * F.max_chunks + c];
-    }
+    for (int i = max(pb, F.prob_int_off[p]) + threadIdx.x; i < pe; i += kNuThreads)
+      add_node_rt(F, F.prob_nodes[i], C, rt);
   }
   g = block_sum<kNuThreads>(g, sh);
   h = block_sum<kNuThreads>(h, sh);
@@ -1737,10 +1739,7 @@ __global__ void __launch_bounds__(kNuSolveThreads) k_nu_solve(DevForest F, const
     } else {
       const int C = (F.tree_nsites[F.prob_tree_off[p]] + F.chunk_w - 1) / F.chunk_w;
       double rt = 0.0;
-      for (int i = F.prob_int_off[p] + threadIdx.x; i < e; i += kNuSolveThreads) {
-        const int64_t v = F.prob_nodes[i];
-        for (int c = 0; c < C; ++c) rt += F.rt_part[v * F.max_chunks + c];
-      }
+      for (int i = F.prob_int_off[p] + threadIdx.x; i < e; i += kNuSolveThreads) add_node_rt(F, F.prob_nodes[i], C, rt);
       r = block_sum<kNuSolveThreads>(rt, sh);
     }
     double phi_star = r / (F.prob_nobs[p] + r);
