@@ -326,3 +326,31 @@ def test_chains_multi_tree_problem_and_heavy_dropout():
         F.close()
     for a, b in zip(*res):
         np.testing.assert_array_equal(a, b)
+
+
+def test_em_batched_problems_equal_each_alone():
+    """Four problems of one forest (the multi-block nu solve, whose launches
+    after a problem converges or finishes its EM exit early) converge at
+    different EM steps; every problem gets the results of a forest of its
+    own, bit for bit (level passes, one chunk per wavefront)."""
+    probs = [_problem("yule", 200 + 150 * i, 90 - 15 * i, 1700 + i) for i in range(4)]
+    dev = torch.cuda.current_device()
+
+    def run(ps):
+        F = Forest([sim.problem_spec(t, q, c) for (t, q, c, _) in ps], device=dev)
+        F.set_pass_mode(2)
+        H = F.num_levels
+        F.set_shapes(np.ones(H, dtype=np.int32), np.ones(H, dtype=np.int32), H)
+        d, nu, phi = _params(F, ps, nu=0.5, phi=0.5, start=0.05)
+        nll, it = F.em(d, nu, phi, 60, 1e-7)
+        out = (nll.cpu().numpy().copy(), it.cpu().numpy().copy(), F.level_to_tree_lengths(d.cpu().numpy()),
+               nu.cpu().numpy().copy(), phi.cpu().numpy().copy())
+        F.close()
+        return out
+
+    nll, it, lens, nu, phi = run(probs)
+    assert len(set(it.tolist())) > 1, it
+    for i, p in enumerate(probs):
+        n1, i1, l1, nu1, phi1 = run([p])
+        assert nll[i] == n1[0] and it[i] == i1[0] and nu[i] == nu1[0] and phi[i] == phi1[0], i
+        np.testing.assert_array_equal(lens[i], l1[0])
