@@ -154,25 +154,34 @@ def autotune(F, d, nu, phi, reps=3, max_levels=8, verbose=False):
         ms = measure(plan, tag)
         if ms < best_ms:
             best, best_ms = plan, ms
-    # the outside chain-region top, then inside chains on top of the plan
+    # the outside chain-region top, then inside chains on top of the plan.
+    # The plans differ by a few per cent, about the spread of one timing, and
+    # the incumbent's time is the least of many: it is timed afresh and every
+    # plan here keeps the lesser of two timings.
     if rows and best.mode != PASS_TREES and not best.fuse:
+        def measure2(plan, tag):
+            return min(measure(plan, tag), measure(plan, tag))
+
+        best_ms = measure2(best, "incumbent, timed again")
         trials = []
         if best.chains & CHAINS_OUTSIDE:
             trials += [(best.chains, (best.chain_tops[0], t)) for t in (0, 12, 10, 8, 6) if t != best.chain_tops[1]]
         for tag_tops in trials:
             trial = best.copy()
             trial.chains, trial.chain_tops = tag_tops
-            ms = measure(trial, "chain_tops=%s" % (trial.chain_tops,))
+            ms = measure2(trial, "chain_tops=%s" % (trial.chain_tops,))
             if ms < best_ms * 0.995:
                 best, best_ms = trial, ms
         for t in (4, 6):
             trial = best.copy()
             trial.chains = best.chains | CHAINS_INSIDE
             trial.chain_tops = (t, best.chain_tops[1])
-            ms = measure(trial, "inside chains, top %d" % t)
+            ms = measure2(trial, "inside chains, top %d" % t)
             if ms < best_ms * 0.995:
                 best, best_ms = trial, ms
     if best.mode in (PASS_LEVELS, PASS_HYBRID) and rows:
+        # the trials below are single timings: so is the incumbent's
+        best_ms = measure(best, "incumbent, timed again")
         sizes = np.diff(F.level_off)
         first = 2 if best.fuse else 1  # the cherry level has no launch of its own when fused
         big = [h for h in np.argsort(-sizes) if first <= h < min(H, best.top_level)][:max_levels]
