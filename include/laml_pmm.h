@@ -324,6 +324,20 @@ int laml_forest_set_heights(laml_forest* f, const double* heights_host);
  * point in both failure cases), host int32 [num_problems]. */
 int laml_forest_ultra_status(laml_forest* f, int32_t* status_host);
 
+/* Per-problem status of the last M-step each problem ran (by laml_mstep or
+ * inside laml_em), host int32 [num_problems]: 0 = both subproblems solved;
+ * bit LAML_MSTEP_FAILED_D = a branch length came out non-finite, or the
+ * ultrametric branch solve hit a singular Newton system; bit
+ * LAML_MSTEP_FAILED_NU = the nu solve did not converge within its evaluation
+ * cap (nu is left at its last iterate).  Problems that have not run an
+ * M-step read 0.
+ * Replaces: the (success, status) pair of EM_solver.Mstep, whose status
+ * strings "failed_d" / ",failed_nu" (EM_solver.py:441-459) EM_optimization
+ * returns from its last M-step (EM_solver.py:490-512). */
+#define LAML_MSTEP_FAILED_D 1
+#define LAML_MSTEP_FAILED_NU 2
+int laml_forest_mstep_status(laml_forest* f, int32_t* status_host);
+
 /* Text of the last error on this thread ("" if none). */
 const char* laml_last_error(void);
 

@@ -12,6 +12,8 @@ LIB_PATH = os.path.join(_HERE, "liblaml_pmm.so")
 LAML_OK = 0
 LAML_OPT_PHI = 1
 LAML_OPT_NU = 2
+LAML_MSTEP_FAILED_D = 1
+LAML_MSTEP_FAILED_NU = 2
 PASS_LEVELS = 0
 PASS_TREES = 1
 PASS_HYBRID = 2
@@ -112,6 +114,7 @@ SIGNATURES = {
     "laml_forest_get_shapes": (ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p]),
     "laml_forest_set_shapes": (ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, ctypes.c_int32]),
     "laml_forest_ultra_status": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
+    "laml_forest_mstep_status": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
     "laml_last_error": (ctypes.c_char_p, []),
     "laml_version": (ctypes.c_char_p, []),
 }

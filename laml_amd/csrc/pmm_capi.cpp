@@ -983,6 +983,28 @@ int laml_forest_ultra_status(laml_forest* f, int32_t* status_host) {
   return LAML_OK;
 }
 
+int laml_forest_mstep_status(laml_forest* f, int32_t* status_host) {
+  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
+  if (!f || !status_host) return fail(LAML_EINVAL, "null argument");
+  DeviceGuard guard(f->device);
+  const int P = f->F.num_problems;
+  std::vector<laml::NuState> st(P);
+  LAML_HIP(hipMemcpy(st.data(), f->F.nu_state, sizeof(laml::NuState) * P, hipMemcpyDeviceToHost));
+  std::vector<int32_t> us;
+  if (f->ultra_on) {
+    us.resize(P);
+    LAML_HIP(hipMemcpy(us.data(), f->U.status, sizeof(int32_t) * P, hipMemcpyDeviceToHost));
+  }
+  for (int p = 0; p < P; ++p) {
+    int32_t s = st[p].status & 3;
+    // a singular Newton system of the ultrametric branch solve is a failed
+    // branch solve; a stalled line search ends at the best point found
+    if (f->ultra_on && us[p] == 2) s |= LAML_MSTEP_FAILED_D;
+    status_host[p] = s;
+  }
+  return LAML_OK;
+}
+
 static int check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(LAML_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));

@@ -1581,6 +1581,9 @@ __global__ void __launch_bounds__(64) k_branch(DevForest F, const double* __rest
     return;
   }
   const double dn = solve_branch(o, F.dmin, F.dmax, d[v]);
+  // non-finite counts give a non-finite length: reported as the reference's
+  // failed branch solve (EM_solver.py:415-418,450-459)
+  if (!isfinite(dn)) F.nu_state[p].d_bad = 1;
   d[v] = dn;
   F.nu_a[pos] = s[0] + s[1] + s[3];
   F.nu_b[pos] = s[2] + s[4];
@@ -1595,8 +1598,64 @@ __global__ void __launch_bounds__(64) k_branch(DevForest F, const double* __rest
 // its partial (h', h''), and the last block of the problem to arrive reduces
 // the partials in block order (deterministic) and takes a bracketed Newton
 // step.  The first launch also sums the dropout responsibilities for phi.
+// A problem still unconverged after the last of these launches is finished
+// by k_nu_solve in resume mode (one block per problem).
 constexpr int kNuThreads = 256;
-constexpr int kNuIters = 8;
+constexpr int kNuIters = 5;
+// evaluations a nu solve may take in all before its M-step reports failure:
+// the bracket halves at least every other evaluation, so this is far more
+// than [0, 1] needs to close to 1e-12 relative at any optimum the clamped
+// counts allow (nu* >= ~1e-12)
+constexpr int kNuMaxEvals = 200;
+
+// One bracketed Newton step on x h'(x), which is nearly linear in x, from the
+// evaluation (G, Hs) = (h'(x), h''(x)).  [lo, hi] brackets the root of h'
+// (hi > 1 until the bound x = 1 has been evaluated).  Returns true when the
+// solve has converged; xn is the next iterate (the answer when converged).
+__device__ __forceinline__ bool nu_newton_step(double x, double G, double Hs, double& lo, double& hi, double& xn) {
+  if (G > 0.0) lo = x; else if (G < 0.0) hi = x;
+  if (G > 0.0 && x >= 1.0) {
+    xn = 1.0;  // optimum at the upper bound
+    return true;
+  }
+  if (G == 0.0) {
+    xn = x;
+    return true;
+  }
+  const double step = x * G / (G + x * Hs);
+  const double top = fmin(hi, 1.0);
+  // the sum over branches carries ~1e-14 relative rounding noise near the
+  // root; a Newton step below 1e-12 x is converged
+  if (fabs(step) <= 1e-12 * x) {
+    xn = fmin(fmax(x - step, 0.0), 1.0);
+    return true;
+  }
+  xn = x - step;
+  if (xn >= 1.0 && hi > 1.0) xn = 1.0;
+  else if (!(xn > lo && xn < top)) xn = 0.5 * (lo + top);
+  return top - lo <= 1e-12 * top;
+}
+
+// One thread's share (stride kNuThreads from branch position i0) of h'(x)
+// and h''(x) over the branch positions [i0 - lane offset, pe) of a span.
+// Every nu kernel sums through these two functions and in the same span
+// order, so all of them give a problem the same nu bit for bit, whichever
+// kernel runs and whatever else the forest holds.
+__device__ __forceinline__ void nu_span_terms(const DevForest& F, int pb, int pe, int lt, double x, double& g,
+                                              double& h) {
+  for (int i = pb + lt; i < pe; i += kNuThreads) {
+    const double bb = F.nu_b[i];
+    if (bb == 0.0) continue;
+    const double dv = F.nu_d[i];
+    const double en = expm1(x * dv);
+    const double ri = 1.0 / en;
+    g += dv * fma(bb, ri, -F.nu_a[i]);
+    h -= bb * dv * dv * (en + 1.0) * (ri * ri);
+  }
+}
+__device__ __forceinline__ void rt_span(const DevForest& F, int pb, int pe, int lt, int C, double& rt) {
+  for (int i = pb + lt; i < pe; i += kNuThreads) add_node_rt(F, F.prob_nodes[i], C, rt);
+}
 
 __global__ void __launch_bounds__(kNuThreads) k_nu_eval(DevForest F, const double* __restrict__ rtilde_in,
                                                         double* __restrict__ nu, double* __restrict__ phi,
@@ -1631,21 +1690,10 @@ __global__ void __launch_bounds__(kNuThreads) k_nu_eval(DevForest F, const doubl
   // first evaluation point: the current nu (or 1/2 when outside (0, 1))
   const double x = (iter == 0) ? ((nu[p] > 0.0 && nu[p] < 1.0) ? nu[p] : 0.5) : st->x;
   double g = 0.0, h = 0.0, rt = 0.0;
-  if (opt_nu) {
-    for (int i = pb + threadIdx.x; i < pe; i += kNuThreads) {
-      const double bb = F.nu_b[i];
-      if (bb == 0.0) continue;
-      const double dv = F.nu_d[i];
-      const double en = expm1(x * dv);
-      const double ri = 1.0 / en;
-      g += dv * fma(bb, ri, -F.nu_a[i]);
-      h -= bb * dv * dv * (en + 1.0) * (ri * ri);
-    }
-  }
+  if (opt_nu) nu_span_terms(F, pb, pe, threadIdx.x, x, g, h);
   if (iter == 0 && (flags & 1u) && !rtilde_in) {
     const int C = (F.tree_nsites[F.prob_tree_off[p]] + F.chunk_w - 1) / F.chunk_w;
-    for (int i = max(pb, F.prob_int_off[p]) + threadIdx.x; i < pe; i += kNuThreads)
-      add_node_rt(F, F.prob_nodes[i], C, rt);
+    rt_span(F, max(pb, F.prob_int_off[p]), pe, threadIdx.x, C, rt);
   }
   g = block_sum<kNuThreads>(g, sh);
   h = block_sum<kNuThreads>(h, sh);
@@ -1678,117 +1726,146 @@ __global__ void __launch_bounds__(kNuThreads) k_nu_eval(DevForest F, const doubl
       if (fabs(phi_star) < F.prob_phimin[p]) phi_star = 0.0;
     }
     phi[p] = phi_star;
+    st->status = st->d_bad ? 1 : 0;
+    st->d_bad = 0;
     st->lo = 0.0;
     st->hi = 2.0;  // > 1: the bound x = 1 has not been evaluated
     st->fin = opt_nu ? 0 : 1;
     if (!opt_nu) return;
   }
-  // bracketed Newton step on x h'(x), which is nearly linear in x
-  double lo = st->lo, hi = st->hi;
-  if (G > 0.0) lo = x; else if (G < 0.0) hi = x;
-  double xn;
-  int fin = 0;
-  if (G > 0.0 && x >= 1.0) {
-    xn = 1.0;  // optimum at the upper bound
-    fin = 1;
-  } else if (G == 0.0) {
-    xn = x;
-    fin = 1;
-  } else {
-    const double step = x * G / (G + x * Hs);
-    const double top = fmin(hi, 1.0);
-    // the sum over branches carries ~1e-14 relative rounding noise near the
-    // root; a Newton step below 1e-12 x is converged
-    if (fabs(step) <= 1e-12 * x) {
-      xn = fmin(fmax(x - step, 0.0), 1.0);
-      fin = 1;
-    } else {
-      xn = x - step;
-      if (xn >= 1.0 && hi > 1.0) xn = 1.0;
-      else if (!(xn > lo && xn < top)) xn = 0.5 * (lo + top);
-      fin = (top - lo <= 1e-12 * top) ? 1 : 0;
-    }
-  }
+  double lo = st->lo, hi = st->hi, xn;
+  const bool fin = nu_newton_step(x, G, Hs, lo, hi, xn);
   st->lo = lo;
   st->hi = hi;
   st->x = xn;
-  st->fin = fin;
+  st->fin = fin ? 1 : 0;
   nu[p] = xn;
 }
 
 // The same phi update and nu solve with one block per problem and every
 // Newton evaluation in one launch (forests of several problems of up to
-// kNuSolveMaxNodes branches): the block sweeps the problem's branches per
-// evaluation and all threads take the same bracketed step.
+// kNuSolveMaxNodes branches).  The block's four groups of kNuThreads threads
+// sum four of the problem's spans (those of k_nu_eval's blocks) at a time,
+// each span exactly as a k_nu_eval block sums it, and the span sums are added
+// in span order, so the result equals k_nu_eval's bit for bit.  resume = 1
+// continues the solves that the k_nu_eval launches left unconverged (from
+// their Newton state; phi is already done) and does nothing for the others.
+// The M-step status of each problem is final after this kernel.
 constexpr int kNuSolveThreads = 1024;
 constexpr int kNuSolveMaxNodes = 65536;
+constexpr int kNuGroups = kNuSolveThreads / kNuThreads;
+
+// sums over the spans of problem p in k_nu_eval's order: (h', h'') at x when
+// want_g, the dropout sum when want_rt
+__device__ void nu_eval_spans(const DevForest& F, int p, double x, bool want_g, bool want_rt, double& G, double& Hs,
+                              double& RT, double* sh) {
+  const int nparts = F.prob_blk_off[p + 1] - F.prob_blk_off[p];
+  const int b = F.prob_node_off[p], e = F.prob_node_off[p + 1];
+  const int span = (e - b + nparts - 1) / nparts;
+  const int grp = threadIdx.x / kNuThreads, lt = threadIdx.x % kNuThreads;
+  const int wid = threadIdx.x >> 6;
+  const int C = want_rt ? (F.tree_nsites[F.prob_tree_off[p]] + F.chunk_w - 1) / F.chunk_w : 0;
+  constexpr int kW = kNuThreads / 64;  // waves per span
+  G = 0.0;
+  Hs = 0.0;
+  RT = 0.0;
+  for (int q0 = 0; q0 < nparts; q0 += kNuGroups) {
+    const int part = q0 + grp;
+    double g = 0.0, h = 0.0, rt = 0.0;
+    if (part < nparts) {
+      const int pb = b + part * span, pe = min(e, pb + span);
+      if (want_g) nu_span_terms(F, pb, pe, lt, x, g, h);
+      if (want_rt) rt_span(F, max(pb, F.prob_int_off[p]), pe, lt, C, rt);
+    }
+    g = seg_sum<64>(g);
+    h = seg_sum<64>(h);
+    rt = seg_sum<64>(rt);
+    if ((threadIdx.x & 63) == 0) {
+      sh[wid] = g;
+      sh[kNuSolveThreads / 64 + wid] = h;
+      sh[2 * (kNuSolveThreads / 64) + wid] = rt;
+    }
+    __syncthreads();
+    for (int j = 0; j < kNuGroups && q0 + j < nparts; ++j) {
+      double sg = 0.0, shh = 0.0, sr = 0.0;
+#pragma unroll
+      for (int w = 0; w < kW; ++w) {
+        sg += sh[j * kW + w];
+        shh += sh[kNuSolveThreads / 64 + j * kW + w];
+        sr += sh[2 * (kNuSolveThreads / 64) + j * kW + w];
+      }
+      G += sg;
+      Hs += shh;
+      RT += sr;
+    }
+    __syncthreads();
+  }
+}
 
 __global__ void __launch_bounds__(kNuSolveThreads) k_nu_solve(DevForest F, const double* __restrict__ rtilde_in,
                                                              double* __restrict__ nu, double* __restrict__ phi,
-                                                             const int32_t* __restrict__ done) {
-  __shared__ double sh[kNuSolveThreads / 64];
+                                                             const int32_t* __restrict__ done, int resume) {
+  __shared__ double sh[3 * (kNuSolveThreads / 64)];
   const int p = blockIdx.x;
+  NuState* st = F.nu_state + p;
   if (done && done[p]) return;
+  // every thread reads the same flag, written by an earlier launch
+  if (resume && st->fin) return;
   const uint32_t flags = F.prob_flags[p];
   const bool opt_nu = (flags & 2u) && F.prob_nfree[p] > 0;
-  const int b = F.prob_node_off[p], e = F.prob_node_off[p + 1];
-  if (flags & 1u) {
-    double r;
-    if (rtilde_in) {
-      r = rtilde_in[p];
-    } else {
-      const int C = (F.tree_nsites[F.prob_tree_off[p]] + F.chunk_w - 1) / F.chunk_w;
-      double rt = 0.0;
-      for (int i = F.prob_int_off[p] + threadIdx.x; i < e; i += kNuSolveThreads) add_node_rt(F, F.prob_nodes[i], C, rt);
-      r = block_sum<kNuSolveThreads>(rt, sh);
+  int status = 0;
+  double x, lo, hi;
+  int evals;
+  bool fin = false;
+  if (!resume) {
+    x = (nu[p] > 0.0 && nu[p] < 1.0) ? nu[p] : 0.5;
+    lo = 0.0;
+    hi = 2.0;  // hi > 1: the bound x = 1 has not been evaluated
+    // the first evaluation also sums the dropout responsibilities, as
+    // k_nu_eval's first launch does
+    const bool want_rt = (flags & 1u) && !rtilde_in;
+    double G, Hs, RT;
+    nu_eval_spans(F, p, x, opt_nu, want_rt, G, Hs, RT, sh);
+    if (flags & 1u) {
+      const double r = rtilde_in ? rtilde_in[p] : RT;
+      double phi_star = r / (F.prob_nobs[p] + r);
+      if (fabs(phi_star) < F.prob_phimin[p]) phi_star = 0.0;
+      if (threadIdx.x == 0) phi[p] = phi_star;
     }
-    double phi_star = r / (F.prob_nobs[p] + r);
-    if (fabs(phi_star) < F.prob_phimin[p]) phi_star = 0.0;
-    if (threadIdx.x == 0) phi[p] = phi_star;
-  }
-  if (!opt_nu) return;
-  double x = (nu[p] > 0.0 && nu[p] < 1.0) ? nu[p] : 0.5;
-  double lo = 0.0, hi = 2.0;  // hi > 1: the bound x = 1 has not been evaluated
-  for (int iter = 0; iter <= kNuIters; ++iter) {
-    double g = 0.0, h = 0.0;
-    for (int i = b + threadIdx.x; i < e; i += kNuSolveThreads) {
-      const double bb = F.nu_b[i];
-      if (bb == 0.0) continue;
-      const double dv = F.nu_d[i];
-      const double en = expm1(x * dv);
-      const double ri = 1.0 / en;
-      g += dv * fma(bb, ri, -F.nu_a[i]);
-      h -= bb * dv * dv * (en + 1.0) * (ri * ri);
+    status = st->d_bad ? 1 : 0;
+    __syncthreads();  // every thread has read d_bad before thread 0 clears it
+    if (threadIdx.x == 0) {
+      st->d_bad = 0;
+      st->status = status;
+      st->fin = 1;
     }
-    const double G = block_sum<kNuSolveThreads>(g, sh);
-    const double Hs = block_sum<kNuSolveThreads>(h, sh);
-    // bracketed Newton step on x h'(x), as in k_nu_eval
-    if (G > 0.0) lo = x; else if (G < 0.0) hi = x;
-    bool fin = false;
+    if (!opt_nu) return;
     double xn;
-    if (G > 0.0 && x >= 1.0) {
-      xn = 1.0;
-      fin = true;
-    } else if (G == 0.0) {
-      xn = x;
-      fin = true;
-    } else {
-      const double step = x * G / (G + x * Hs);
-      const double top = fmin(hi, 1.0);
-      if (fabs(step) <= 1e-12 * x) {
-        xn = fmin(fmax(x - step, 0.0), 1.0);
-        fin = true;
-      } else {
-        xn = x - step;
-        if (xn >= 1.0 && hi > 1.0) xn = 1.0;
-        else if (!(xn > lo && xn < top)) xn = 0.5 * (lo + top);
-        fin = top - lo <= 1e-12 * top;
-      }
-    }
+    fin = nu_newton_step(x, G, Hs, lo, hi, xn);
     x = xn;
-    if (fin) break;
+    evals = 1;
+  } else {
+    x = st->x;
+    lo = st->lo;
+    hi = st->hi;
+    evals = kNuIters + 1;
+    status = st->status;
   }
-  if (threadIdx.x == 0) nu[p] = x;
+  for (; evals < kNuMaxEvals && !fin; ++evals) {
+    double G, Hs, RT;
+    nu_eval_spans(F, p, x, true, false, G, Hs, RT, sh);
+    double xn;
+    fin = nu_newton_step(x, G, Hs, lo, hi, xn);
+    x = xn;
+  }
+  if (threadIdx.x == 0) {
+    nu[p] = x;
+    st->x = x;
+    st->lo = lo;
+    st->hi = hi;
+    st->fin = 1;
+    st->status = status | (fin ? 0 : 2);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2668,11 +2745,13 @@ void run_mstep(const DevForest& F, const double* counts, const double* rtilde, d
   // one launch when the problems keep enough blocks busy; otherwise split
   // each problem's branches over several blocks per evaluation
   if (F.max_prob_nodes <= kNuSolveMaxNodes && F.num_problems >= 8) {
-    hipLaunchKernelGGL(k_nu_solve, dim3(F.num_problems), dim3(kNuSolveThreads), 0, s, F, rtilde, nu, phi, done);
+    hipLaunchKernelGGL(k_nu_solve, dim3(F.num_problems), dim3(kNuSolveThreads), 0, s, F, rtilde, nu, phi, done, 0);
     return;
   }
   for (int it = 0; it <= kNuIters; ++it)
     hipLaunchKernelGGL(k_nu_eval, dim3(F.num_nu_blocks), dim3(kNuThreads), 0, s, F, rtilde, nu, phi, done, it);
+  // solves the launches above left unconverged (rare): one block each
+  hipLaunchKernelGGL(k_nu_solve, dim3(F.num_problems), dim3(kNuSolveThreads), 0, s, F, rtilde, nu, phi, done, 1);
 }
 
 }  // namespace laml

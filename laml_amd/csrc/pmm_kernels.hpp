@@ -9,11 +9,15 @@
 
 namespace laml {
 
-// Newton state of one problem's nu solve
+// Newton state of one problem's nu solve, and the status of its last M-step
 struct NuState {
   double x, lo, hi;
   int32_t fin;
-  int32_t count;  // blocks arrived in the current evaluation
+  int32_t count;   // blocks arrived in the current evaluation
+  int32_t status;  // last M-step: bit 0 a branch length came out non-finite,
+                   // bit 1 the nu solve did not converge (kept at its last
+                   // iterate)
+  int32_t d_bad;   // set by k_branch, folded into status by the nu solve
 };
 
 // Per internal node, what the inside row kernel reads before its row loads

@@ -422,6 +422,13 @@ class Forest(ForestLayout):
         _lib.check(self._lib.laml_forest_ultra_status(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return out
 
+    def mstep_status(self):
+        """Per-problem status bits of the last M-step (LAML_MSTEP_FAILED_D = 1,
+        LAML_MSTEP_FAILED_NU = 2; 0 = solved)."""
+        out = np.zeros(self.num_problems, dtype=np.int32)
+        _lib.check(self._lib.laml_forest_mstep_status(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return out
+
     def labels(self):
         """az-partition labels: (leaf rows, internal rows) int16 flat arrays in
         level order, row lengths K of the node's tree."""

@@ -20,7 +20,7 @@ from math import isfinite
 import numpy as np
 import torch
 
-from ._lib import LAML_OPT_NU, LAML_OPT_PHI
+from ._lib import LAML_MSTEP_FAILED_D, LAML_MSTEP_FAILED_NU, LAML_OPT_NU, LAML_OPT_PHI
 from .forest import DMAX, DMIN, Forest, ProblemSpec, SiteTable, resolved_tree_arrays, tree_arrays_from_tree
 from .lca import find_LCAs
 from .tree import read_tree_newick
@@ -36,6 +36,21 @@ DEFAULT_STRATEGY = {'resolve_search_only': False, 'only_marked': False, 'ultra_c
                     'fixed_phi': None, 'fixed_nu': None, 'local_brlen_opt': True,
                     'nodes_to_recompute': None}
 
+
+
+def mstep_status(bits):
+    """(success, status) of EM_solver.Mstep from the device's M-step status
+    bits (EM_solver.py:450-459: "failed_d" when the branch solve failed, and
+    ",failed_nu" -- replacing it -- when the nu solve failed)."""
+    bits = int(bits)
+    if bits == 0:
+        return True, "optimal"
+    status = ""
+    if bits & LAML_MSTEP_FAILED_D:
+        status += "failed_d"
+    if bits & LAML_MSTEP_FAILED_NU:
+        status = ",failed_nu"
+    return False, status
 
 class Params:
     def __init__(self, nu, phi):
@@ -320,7 +335,7 @@ class EM_solver:
             self._disable_ultra(F)
         F.mstep(d, nu, phi)
         self._store_params(F, d, nu, phi)
-        return True, "optimal"
+        return mstep_status(F.mstep_status()[0])
 
     def EM_optimization(self, verbose=1, optimize_phi=True, optimize_nu=True, ultra_constr=False, maxIter=1000):
         """EM_solver.EM_optimization (EM_solver.py:461-512), whole loop on
@@ -343,9 +358,13 @@ class EM_solver:
         self._store_params(F, d, nu, phi)
         nllh = float(nll.item())
         em_iter = int(iters.item())
+        # the status of the last M-step, as the reference's loop returns it
+        ok, status = mstep_status(F.mstep_status()[0])
+        if not ok and verbose >= 0:
+            print("Warning: EM failed to optimize parameters in one Mstep.")
         if em_iter > maxIter and verbose >= 0:
             print("Warning: exceeded maximum number of EM iterations (" + str(maxIter) + " iters)!")
-        return nllh, em_iter, "optimal"
+        return nllh, em_iter, status
 
     # ---------------- initial points (ML_solver.py:195-247) ----------------
     def ini_brlens(self):

@@ -25,7 +25,7 @@ import torch
 from ._lib import LAML_OPT_NU, LAML_OPT_PHI
 from .forest import Forest
 from .lca import find_LCAs
-from .solvers import CONV_EPS, DEFAULT_STRATEGY, EM_solver
+from .solvers import CONV_EPS, DEFAULT_STRATEGY, EM_solver, mstep_status
 from .tree import read_tree_newick
 
 CHKPT_FREQ = 10  # laml_libs/core_constants.py:9
@@ -314,6 +314,7 @@ def score_batch(jobs, data, prior, params, solver_cls=EM_solver, max_iter=1000, 
         phi = torch.tensor([float(s.params.phi) for (_, s, _) in batch], dtype=torch.float64, device=F.device)
         nll, iters = F.em(d, nu, phi, max_iter, tol)
         nll = nll.cpu().numpy()
+        mst = F.mstep_status()
         nu_h, phi_h = nu.cpu().numpy(), phi.cpu().numpy()
         per_tree = F.level_to_tree_lengths(d.cpu().numpy())
         ti = 0
@@ -336,7 +337,7 @@ def score_batch(jobs, data, prior, params, solver_cls=EM_solver, max_iter=1000, 
                     v.polytomy_mark = False
                     v.mark_fixed = False
             s._forest = None
-            out[j] = (s, -float(nll[k]), "optimal")
+            out[j] = (s, -float(nll[k]), mstep_status(mst[k])[1])
         F.close()
     return out
 

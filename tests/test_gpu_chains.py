@@ -328,12 +328,15 @@ def test_chains_multi_tree_problem_and_heavy_dropout():
         np.testing.assert_array_equal(a, b)
 
 
-def test_em_batched_problems_equal_each_alone():
-    """Four problems of one forest (the multi-block nu solve, whose launches
-    after a problem converges or finishes its EM exit early) converge at
-    different EM steps; every problem gets the results of a forest of its
-    own, bit for bit (level passes, one chunk per wavefront)."""
-    probs = [_problem("yule", 200 + 150 * i, 90 - 15 * i, 1700 + i) for i in range(4)]
+@pytest.mark.parametrize("nprob", [4, 9])
+def test_em_batched_problems_equal_each_alone(nprob):
+    """Problems of one forest converge at different EM steps; every problem
+    gets the results of a forest of its own, bit for bit (level passes, one
+    chunk per wavefront).  Four problems use the multi-block nu solve (whose
+    launches after a problem converges or finishes its EM exit early), nine
+    the one-block solve, which sums in the multi-block solve's order, while
+    each problem alone uses the multi-block solve."""
+    probs = [_problem("yule", 200 + 150 * (i % 4), 90 - 15 * (i % 4) - i, 1700 + i) for i in range(nprob)]
     dev = torch.cuda.current_device()
 
     def run(ps):
