@@ -62,7 +62,10 @@ def parse_tree(tree, has_branch_mask=False, ordered_leaf_labels=None):
         labels.append(v.label)
     index = {id(v): i for i, v in enumerate(pre)}
     parent = [index[id(v.parent)] if v.parent is not None else -1 for v in pre]
-    children = [[index[id(c)] for c in v.children] for v in pre]
+    # a node's children in the order the preorder reaches them (last child
+    # first): the order of its successors in the reference's DiGraph, whose
+    # edges are added as the preorder reaches each child
+    children = [[index[id(c)] for c in reversed(v.children)] for v in pre]
     keep = list(range(len(pre)))
     if len(children[0]) == 1:  # synthetic root with out-degree 1
         keep = keep[1:]

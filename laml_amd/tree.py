@@ -70,14 +70,20 @@ class Node:
                     stack.append((c, False))
 
     def traverse_preorder(self):
+        """treeswift's preorder: a node, then the subtrees of its children
+        from the last child to the first (a stack that pops the last child
+        pushed).  The reference's outputs are in this order: the node lines
+        of its annotation files (runner.py:421-428), e.g.
+        case_tests/test1/example3_annotations.txt lists I0, I1, c, I2, b, a
+        for ((a,b),c)."""
         stack = [self]
         while stack:
             v = stack.pop()
             yield v
-            for c in reversed(v.children):
-                stack.append(c)
+            stack.extend(v.children)
 
     def traverse_leaves(self):
+        """The leaves in preorder (treeswift's traverse_leaves)."""
         for v in self.traverse_preorder():
             if not v.children:
                 yield v
@@ -189,7 +195,13 @@ class Tree:
 
     def collapse_short_branches(self, threshold):
         """Contract internal non-root branches of length <= threshold (None
-        counts as 0); children of a contracted node join its parent."""
+        counts as 0), in postorder: the children of a contracted node are
+        appended to its parent's children and keep their own lengths (the
+        contracted branch's length is dropped).  This is what the reference's
+        writer produced: examples/out_example2_api/*_trees.collapsed.nwk has
+        the contracted nodes' children at the end of their new parent's list,
+        and its scale (height / timescale) is the height with the contracted
+        lengths removed."""
         if threshold < 0:
             raise ValueError("threshold must be non-negative")
         for node in list(self.traverse_postorder()):
@@ -197,14 +209,11 @@ class Tree:
                 node.edge_length = 0.0
             if node.edge_length <= threshold and not node.is_leaf() and not node.is_root():
                 par = node.parent
-                idx = par.children.index(node)
-                par.children.pop(idx)
+                par.children.remove(node)
                 node.parent = None
                 for c in node.children:
-                    if c.edge_length is not None:
-                        c.edge_length += node.edge_length
                     c.parent = par
-                par.children[idx:idx] = node.children
+                par.children.extend(node.children)
                 node.children = []
 
     def extract_tree_without(self, labels, suppress_unifurcations=True):
@@ -226,12 +235,15 @@ class Tree:
         return t
 
     def height(self):
-        """Root-to-tip path length maxima (excluding the stem)."""
+        """Largest distance from the top of the root's branch to a leaf
+        (treeswift's weighted height: the root's own edge length counts).
+        The reference's writer divides by it: its example3 run logs a
+        pre-scaling height of 1.1061115018134888 for a tree whose stem is
+        0.005 and whose leaves sit 1.1011115 below the root."""
         best = 0.0
-        dist = {self.root: 0.0}
+        dist = {}
         for v in self.traverse_preorder():
-            if v is not self.root:
-                dist[v] = dist[v.parent] + (v.edge_length or 0.0)
+            dist[v] = (0.0 if v.parent is None else dist[v.parent]) + (v.edge_length or 0.0)
             if v.is_leaf():
                 best = max(best, dist[v])
         return best

@@ -25,9 +25,11 @@ def test_annotation_text_layout():
         vals[v] = (lab, [lg(0.5)], [lg(0.25)], [0.0, 0.1234, 0.2, 0.0, 0.3])
     text = annotation_text([t], [vals], [{1: 1.0}])
     lines = text.splitlines()
-    # new root I0 above the root; internal nodes named I1, I2 in preorder
+    # new root I0 above the root; internal nodes named I1, I2 in treeswift's
+    # preorder (last child first: I0, I1, c, I2, b, a)
     assert lines[0] == "(((a:0.623,b:0.623)I2:0.623,c:0.623)I1:0.623)I0;"
+    assert [x.split(",")[0] for x in lines[1:]] == ["I0", "I1", "c", "I2", "b", "a"]
     assert lines[1] == "I0,0"
     assert lines[2] == "I1,0:0.5/-1:0.25/1:0.25"
-    assert lines[4] == "a,0:0.5/-1:0.25/z:0.25"
+    assert lines[6] == "a,0:0.5/-1:0.25/z:0.25"
     assert params_text(0.1, 0.2, 3.5, 1.0).splitlines()[2] == "Negative-llh: 3.5"

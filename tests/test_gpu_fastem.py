@@ -59,7 +59,11 @@ def test_device_estep_on_fastem_model_matches_dense_oracle():
     nwk, msa_r, Qr = _dense_inputs(T, p, phy)
     # the dense oracle reads float32 priors exactly as the device does
     want_nll, want_counts, want_rt = dense_estep(nwk, msa_r, Qr, 0.15, 0.2, missing=("?", -1), silenced=())
-    got = counts.view(-1, 5).cpu().numpy()[F.inv]
+    # device rows by the fastEM tree's post-order (its children lists) ->
+    # node labels -> the dense oracle's post-order over the same tree
+    got_post = counts.view(-1, 5).cpu().numpy()[F.inv]
+    by_label = {p["relabeling_vector"][v]: got_post[i] for i, v in enumerate(post)}
+    got = np.asarray([by_label[v.label] for v in T.traverse_postorder()])
     print("fastEM model", nll.item(), want_nll, flush=True)
     assert math.isclose(nll.item(), want_nll, rel_tol=1e-9)
     np.testing.assert_allclose(got, want_counts, rtol=1e-9, atol=1e-10)

@@ -89,10 +89,12 @@ def test_dedup_and_add_dup():
 
 
 def test_tree_writers():
-    text, rate = io.scaled_trees_text(["((a:1,b:3):1,c:2):0.5;"], timescale=2.0)
-    assert rate == 2.0  # height 4 over timescale 2
-    assert text == "(((a:0.5,b:1.5):0.5,c:1):0.25);\n"
-    # the 0.004 branch (<= 1.2 dmin) is collapsed, then everything scaled by height 3.004
+    # height 5 (the root's own branch counts) over timescale 2
+    text, rate = io.scaled_trees_text(["((a:1,b:3):1.5,c:2):0.5;"], timescale=2.0)
+    assert rate == 2.5
+    assert text == "(((a:0.4,b:1.2):0.6,c:0.8):0.2);\n"
+    # the 0.004 branch (<= 1.2 dmin) is contracted (its length dropped), then
+    # everything is scaled by height 3.5
     col = io.collapsed_trees_text(["((a:1,b:3):0.004,c:2):0.5;"], timescale=1.0)
     t = read_tree_newick(col.strip()[1:-2] + ";")
     assert sorted(v.label for v in t.root.children) == ["a", "b", "c"]
