@@ -427,18 +427,8 @@ def bench_scoring(args):
     from laml_amd.forest import Forest
 
     c = SCORING[args.workload]
-    rng = np.random.Generator(np.random.PCG64(c["seed"]))
-    base_tree, Q, codes, _ = sim.make_problem(c["base_n"], c["K"], c["m_lo"], c["m_hi"], c["alpha"], c["nu"],
-                                              c["phi"], c["seed"])
-    problems = []
-    for i in range(c["variants"]):
-        t, cd = sim.nni_variant(base_tree, codes, rng)
-        problems.append((t, Q, cd))
-    sizes = np.exp(rng.uniform(np.log(c["n_lo"]), np.log(c["n_hi"]), c["others"])).astype(int)
-    for i, n in enumerate(sizes):
-        t, q, cd, _ = sim.make_problem(int(n), c["K"], c["m_lo"], c["m_hi"], c["alpha"], c["nu"], c["phi"],
-                                       c["seed"] + 1 + i)
-        problems.append((t, q, cd))
+    problems = sim.scoring_workload(**c)
+    sizes = np.asarray([(t.n_nodes + 1) // 2 for (t, _, _) in problems[c["variants"]:]])
     P = len(problems)
     log("scoring workload %s: %d trees (%d NNI variants of a %d-leaf tree + %d trees of %d-%d leaves), K=%d" % (
         args.workload, P, c["variants"], c["base_n"], c["others"], sizes.min(), sizes.max(), c["K"]))

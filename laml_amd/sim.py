@@ -246,3 +246,22 @@ def nni_variant(tree, codes, rng):
     row = {name: i for i, name in enumerate(tree.leaf_names)}
     perm = np.asarray([row[nm] for nm in names], dtype=np.int64)
     return YuleTree(c0, c1, parent, length, names), codes[perm]
+
+
+def scoring_workload(base_n, variants, others, n_lo, n_hi, K, m_lo, m_hi, alpha, nu, phi, seed):
+    """Batched-scoring workload: `variants` NNI-perturbed copies of one seeded
+    `base_n`-leaf Yule tree (sharing its characters and priors) followed by
+    `others` independent Yule trees with log-uniform leaf counts in
+    [n_lo, n_hi], each with its own priors and characters, all with K sites.
+    Returns a list of (YuleTree, Q, codes)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    base_tree, Q, codes, _ = make_problem(base_n, K, m_lo, m_hi, alpha, nu, phi, seed)
+    problems = []
+    for _ in range(variants):
+        t, cd = nni_variant(base_tree, codes, rng)
+        problems.append((t, Q, cd))
+    sizes = np.exp(rng.uniform(np.log(n_lo), np.log(n_hi), others)).astype(int)
+    for i, n in enumerate(sizes):
+        t, q, cd, _ = make_problem(int(n), K, m_lo, m_hi, alpha, nu, phi, seed + 1 + i)
+        problems.append((t, q, cd))
+    return problems

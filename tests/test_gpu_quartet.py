@@ -1,7 +1,11 @@
-"""Quartet-local EM of NNI candidates (laml_quartet_em) against the full
-per-candidate solver the reference builds for every candidate
-(Topology_search.apply_nni with local_brlen_opt): the same NLL, the same
-five optimised branch lengths and the same EM iteration count."""
+"""Quartet-local EM of NNI candidates (laml_quartet_em) against the CPU
+oracle's EM of the same candidate, set up as the reference's apply_nni sets
+it up with local_brlen_opt (Topology_search.py:247-306: the branches outside
+the quartet fixed at the lengths their anchor pairs name, nu and phi fixed,
+ini_brlens raising every branch to 2 dmin, EM_solver.EM_optimization on the
+five free branches): the same NLL (1e-9 relative), the same five optimised
+branch lengths (1e-6) and the same EM iteration count.  The device's full
+per-candidate solver is held to the same three quantities."""
 import math
 import random
 
@@ -14,6 +18,8 @@ from laml_amd.quartet import QuartetScorer, _clade_index
 from laml_amd.solvers import DEFAULT_STRATEGY, EM_solver
 from laml_amd.topology_search import Topology_search_batched, _nni_swap, _nni_unswap
 from laml_amd.tree import read_tree_newick
+from oracle.em_oracle import OracleEM, postorder
+from oracle.pmm_ref import RefProblem
 
 pytestmark = pytest.mark.gpu
 
@@ -54,16 +60,32 @@ def test_quartet_em_equals_full_candidate_em(n, K, seed):
         s = EM_solver(nwks, {"charMtrx": msa}, {"Q": Q}, ts.params)
         nllh, status = s.optimize(initials=1, verbose=-1, fixed_phi=st["fixed_phi"], fixed_nu=st["fixed_nu"],
                                   fixed_brlen=st["fixed_brlen"], random_seeds=[1234])
-        print("move", i, -score, nllh, iters_q[i], s.em_iterations, flush=True)
-        assert math.isclose(-score, nllh, rel_tol=1e-9), (i, -score, nllh)
-        assert abs(int(iters_q[i]) - s.em_iterations) <= 1
+        # the oracle: the same candidate, marked and started as the reference does
+        o = OracleEM(nwks, msa, Q)
+        o.mark_fixed(st["fixed_brlen"])
+        x0 = o.ini_all(random.Random(1234), fixed_phi=st["fixed_phi"], fixed_nu=st["fixed_nu"])
+        o.x2params(x0, fixed_phi=st["fixed_phi"], fixed_nu=st["fixed_nu"])
+        rp, d0 = RefProblem.from_oracle(o)
+        onll, oit, od, _, _ = rp.em(d0, o.nu, o.phi, opt_nu=False, opt_phi=False)
+        print("move", i, -score, nllh, onll, iters_q[i], s.em_iterations, oit, flush=True)
+        assert math.isclose(-score, onll, rel_tol=1e-9), (i, -score, onll)
+        assert math.isclose(nllh, onll, rel_tol=1e-9), (i, nllh, onll)
+        assert int(iters_q[i]) == oit, (i, int(iters_q[i]), oit)
+        assert s.em_iterations == oit, (i, s.em_iterations, oit)
         # the five free branches, by clade in the candidate tree
         A = next(x for x in u.child_nodes() if x is not uc)
         cA, cW, cC = below[id(A)], below[id(w)], below[id(uc)]
         clades = [cA, cW, cA | cW, cC, cA | cW | cC]
         got = lazy.lengths
         want = _lengths_by_clade(s.get_tree_newick()[0])
+        oblow = {}
+        oracle_len = {}
+        for v, x in zip([v for r in o.trees for v in postorder(r)], od):
+            cl = frozenset([v.label]) if not v.children else frozenset().union(*[oblow[id(c)] for c in v.children])
+            oblow[id(v)] = cl
+            oracle_len[cl] = x
         for j, cl in enumerate(clades):
+            assert abs(got[j] - oracle_len[cl]) <= 1e-6 * max(1.0, oracle_len[cl]), (i, j, got[j], oracle_len[cl])
             assert abs(got[j] - want[cl]) <= 1e-6 * max(1.0, want[cl]), (i, j, got[j], want[cl])
         # the accepted-candidate tree text carries the same lengths
         mat = _lengths_by_clade(lazy.materialise().get_tree_newick()[0])
