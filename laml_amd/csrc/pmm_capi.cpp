@@ -1425,6 +1425,8 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
 
 int laml_posteriors(laml_forest* f, const double* phi, double* post0_host, double* postS_host, int64_t capacity,
                     void* stream) {
+  // allocates, frees and synchronises: not while another thread captures
+  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
   if (!f || !phi || !post0_host || !postS_host) return fail(LAML_EINVAL, "null argument");
   const int64_t need = f->leaf_sites + f->int_sites;
   if (capacity < need) return fail(LAML_EINVAL, "posterior capacity too small");
