@@ -267,6 +267,21 @@ int laml_em(laml_forest* f, double* d, double* nu, double* phi, int32_t max_iter
 int laml_posteriors(laml_forest* f, const double* phi, double* post0_host, double* postS_host, int64_t capacity,
                     void* stream);
 
+/* laml_posteriors without any allocation, for callers that cannot allocate
+ * device memory during the call (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same results as
+ * laml_posteriors, bit for bit; the staging rows live in a fixed file-scope
+ * device array of 8388608 doubles (64 MiB) that each device holds from the
+ * library's first forest on it until the process ends.  A forest whose
+ * 2 x (padded rows + leaf_sites + internal sites) exceeds that is refused
+ * with LAML_EINVAL before any work (laml_posteriors has no such limit).
+ * Every call on one device shares the array: calls that overlap in time on
+ * one device (other threads, other streams) race and must be serialised by
+ * the caller.  The call ends with a synchronisation of `stream`.
+ * No counterpart in the reference. */
+int laml_posteriors_noalloc(laml_forest* f, const double* phi, double* post0_host, double* postS_host,
+                            int64_t capacity, void* stream);
+
 /* Quartet-local EM of NNI candidates.  With nu, phi and every branch but
  * the five around an NNI move fixed, the subtrees hanging off the move keep
  * their inside vectors and the data above it its outside state, so each

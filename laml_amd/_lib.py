@@ -109,6 +109,8 @@ SIGNATURES = {
     "laml_forest_chains": (ctypes.c_int32, [ctypes.c_void_p]),
     "laml_posteriors": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p, _f64p, ctypes.c_int64,
                                          ctypes.c_void_p]),
+    "laml_posteriors_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p, _f64p, ctypes.c_int64,
+                                                 ctypes.c_void_p]),
     "laml_quartet_em": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32] + [ctypes.c_void_p] * 6 +
                         [ctypes.c_int32, ctypes.c_double] + [ctypes.c_void_p] * 4),
     "laml_forest_get_shapes": (ctypes.c_int, [ctypes.c_void_p, _i32p, _i32p, _i32p]),

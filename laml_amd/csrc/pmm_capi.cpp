@@ -103,7 +103,10 @@ struct laml_forest {
   std::vector<int32_t> h_Kt;        // sites of each node's tree
   std::vector<int64_t> h_row_off;   // device row offsets (padded)
   std::vector<int64_t> h_desc_off;  // row offsets of the unpadded description layout
+  const int64_t* d_desc_off = nullptr;  // device copy (laml_posteriors_noalloc)
   int64_t leaf_sites = 0, int_sites = 0;
+  // device address of the posterior staging array on this forest's device
+  double* post_scratch = nullptr;
   void* beta_alloc = nullptr;
   void* bE_alloc = nullptr;
   int64_t beta_rows = 0;
@@ -648,6 +651,17 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   f->h_desc_off = D.row_off;
   f->leaf_sites = D.leaf_rows;
   f->int_sites = D.int_rows;
+  if ((rc = f->upload(&f->d_desc_off, f->h_desc_off.data(), (size_t)N))) {
+    delete f;
+    return rc;
+  }
+  // the staging array of laml_posteriors_noalloc: its address on this device
+  // (looked up once per device, with the device current)
+  if (const int se = laml::post_scratch(device, &f->post_scratch)) {
+    delete f;
+    return fail(LAML_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                  hipGetErrorString(static_cast<hipError_t>(se)));
+  }
   f->h_Kt.resize(N);
   for (int v = 0; v < N; ++v) f->h_Kt[v] = d->tree_nsites[d->tree_of[v]];
   F.coef = coef;
@@ -1466,6 +1480,42 @@ int laml_posteriors(laml_forest* f, const double* phi, double* post0_host, doubl
     std::memcpy(post0_host + dst, h0.data() + src, sizeof(double) * K);
     std::memcpy(postS_host + dst, hS.data() + src, sizeof(double) * K);
   }
+  return LAML_OK;
+}
+
+int laml_posteriors_noalloc(laml_forest* f, const double* phi, double* post0_host, double* postS_host,
+                            int64_t capacity, void* stream) {
+  if (!f || !phi || !post0_host || !postS_host) return fail(LAML_EINVAL, "null argument");
+  const int64_t need = f->leaf_sites + f->int_sites;
+  if (capacity < need) return fail(LAML_EINVAL, "posterior capacity too small");
+  const int64_t rows = f->leaf_rows + f->int_rows;
+  // padded rows of both posteriors, then their packed copies
+  if (2 * (rows + need) > laml::kPostScratchDoubles)
+    return fail(LAML_EINVAL, "laml_posteriors_noalloc: the forest's posteriors (2 x (padded rows + entries) "
+                             "doubles) exceed the fixed staging area of 8388608 doubles; use laml_posteriors");
+  if (f->ckpt_L > 0) return fail(LAML_ESTATE, "posteriors need full inside storage (level checkpointing is on)");
+  if (f->inside_fused)
+    return fail(LAML_ESTATE, "posteriors need every node's rows: rerun the E-step with cherry fusion off "
+                             "(laml_forest_set_fusion)");
+  if (!f->post_scratch) return fail(LAML_ESTATE, "no staging area on this device");
+  DeviceGuard guard(f->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevForest F = f->F;
+  F.phi = phi;
+  if (!f->alpha_complete) pass_outside(f, F, true, nullptr, s, false);
+  double* p0 = f->post_scratch;
+  double* pS = p0 + rows;
+  double* o0 = pS + rows;
+  double* oS = o0 + need;
+  laml::run_node_post(F, f->W, f->leaf_rows, p0, pS, s);
+  laml::run_post_pack(F, f->d_desc_off, f->leaf_rows, f->leaf_sites, p0, pS, o0, oS, s);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && need > 0)
+    e = hipMemcpyAsync(post0_host, o0, sizeof(double) * need, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && need > 0)
+    e = hipMemcpyAsync(postS_host, oS, sizeof(double) * need, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(LAML_EDEVICE, std::string("posteriors: ") + hipGetErrorString(e));
   return LAML_OK;
 }
 

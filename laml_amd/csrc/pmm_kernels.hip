@@ -14,6 +14,8 @@
 #include "pmm_kernels.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 namespace laml {
 
@@ -2700,6 +2702,59 @@ void run_node_post(const DevForest& F, int W, int64_t leaf_rows, double* p0, dou
   LAML_DISPATCH_W(W, {
     hipLaunchKernelGGL(k_node_post<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, leaf_rows, p0, pS);
   });
+}
+
+// ---------------------------------------------------------------------------
+// Staging area of laml_posteriors_noalloc: a file-scope device array, so that
+// the entry point needs no allocation.  In host code the name `scratch` is a
+// host-side stand-in, never a device address: the address every kernel
+// argument and copy uses is the one hipGetSymbolAddress returns for the
+// device that is current, looked up once per device by post_scratch().
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kPostScratchDoubles];
+
+int post_scratch(int device, double** out) {
+  static std::mutex mu;
+  static std::map<int, double*> by_device;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = by_device.find(device);
+  if (it == by_device.end()) {
+    // the caller has made `device` current: each device holds its own copy
+    // of the array at its own address
+    double* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    it = by_device.emplace(device, ptr).first;
+  }
+  *out = it->second;
+  return 0;
+}
+
+// Copies each node's K posterior values from the padded row layout of
+// run_node_post (leaf rows, then internal rows at leaf_rows) to the packed
+// layout of laml_forest_labels (leaf entries, then internal entries at
+// leaf_sites; desc_off[v] = the node's first entry in its part).
+__global__ void __launch_bounds__(64) k_post_pack(DevForest F, const int64_t* __restrict__ desc_off,
+                                                  int64_t leaf_rows, int64_t leaf_sites,
+                                                  const double* __restrict__ p0, const double* __restrict__ pS,
+                                                  double* __restrict__ out0, double* __restrict__ outS) {
+  const int v = blockIdx.x;
+  if (v >= F.num_nodes) return;
+  const int K = F.tree_nsites[F.tree_of[v]];
+  const bool leaf = v < F.num_leaves;
+  const int64_t src = (leaf ? 0 : leaf_rows) + F.row_off[v];
+  const int64_t dst = (leaf ? 0 : leaf_sites) + desc_off[v];
+  for (int k = threadIdx.x; k < K; k += 64) {
+    out0[dst + k] = p0[src + k];
+    outS[dst + k] = pS[src + k];
+  }
+}
+
+void run_post_pack(const DevForest& F, const int64_t* desc_off, int64_t leaf_rows, int64_t leaf_sites,
+                   const double* p0, const double* pS, double* out0, double* outS, hipStream_t s) {
+  if (F.num_nodes <= 0) return;
+  hipLaunchKernelGGL(k_post_pack, dim3(F.num_nodes), dim3(64), 0, s, F, desc_off, leaf_rows, leaf_sites, p0, pS,
+                     out0, outS);
 }
 
 void run_counts(const DevForest& F, double* counts, hipStream_t s) {

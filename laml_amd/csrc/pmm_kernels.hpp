@@ -215,6 +215,14 @@ void run_outside_levels(const DevForest& F, const std::vector<int32_t>& lvl, int
 void run_llh(const DevForest& F, double* nll, EmState em, int max_iter, double tol, hipStream_t s);
 void run_em_reset(const EmState& em, int P, hipStream_t s);
 void run_node_post(const DevForest& F, int W, int64_t leaf_rows, double* p0, double* pS, hipStream_t s);
+// doubles in the file-scope staging array of laml_posteriors_noalloc (64 MiB
+// per device, held until the process ends)
+constexpr int64_t kPostScratchDoubles = (int64_t)8 << 20;
+// device address of that array on `device` (which must be current), looked
+// up once per device; 0 or a hipError_t
+int post_scratch(int device, double** out);
+void run_post_pack(const DevForest& F, const int64_t* desc_off, int64_t leaf_rows, int64_t leaf_sites,
+                   const double* p0, const double* pS, double* out0, double* outS, hipStream_t s);
 void run_quartet_em(const DevForest& F, int ncand, const int32_t* qnodes, const double* len_in,
                     const uint8_t* opt_mask, const double* llh_const, const double* nu, const double* phi,
                     int max_iter, double tol, double* len_out, double* nll_out, int32_t* iters_out, hipStream_t s);
