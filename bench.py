@@ -512,6 +512,192 @@ def bench_scoring(args):
     print(json.dumps(res), flush=True)
 
 
+def serving_leg(requests=24, load=2.0, slots=4, cpu_budget_s=6.0):
+    """The latency half of the metric, bounded for the default run: a
+    Poisson stream of `requests` likelihood + EM solves in the serving
+    workload's ranges (leaves log-uniform 200-50,000, sites 20-500, nu in
+    [0.02,0.2], phi in [0.05,0.3], 50 EM iterations, tol 1e-8) offered at
+    `load` x the plain path's capacity, served by the continuous-batching
+    server (`slots` concurrent solves, admission by device footprint) and by
+    the plain per-request path (one solve at a time) on the same arrivals."""
+    import torch
+
+    from laml_amd import sim
+    from laml_amd.serving import PMMServer, SolveRequest, latency_summary, solve_alone
+
+    c = SERVING["serve"]
+    rng = np.random.Generator(np.random.PCG64(c["seed"]))
+    probs = []
+    for i in range(requests):
+        n = int(np.exp(rng.uniform(np.log(c["n_lo"]), np.log(c["n_hi"]))))
+        K = int(rng.integers(c["K_lo"], c["K_hi"] + 1))
+        nu = float(rng.uniform(*c["nu"]))
+        phi = float(rng.uniform(*c["phi"]))
+        probs.append(sim.make_problem(n, K, 5, 50, 1.0, nu, phi, c["seed"] + 1 + i))
+    specs = [sim.problem_spec(t, q, cd) for (t, q, cd, _) in probs]
+
+    def make(i):
+        return SolveRequest(specs[i], [np.full(probs[i][0].n_nodes, 0.05)], 0.5, 0.5, max_iter=c["em_iters"],
+                            tol=c["tol"], rid=i)
+
+    torch.cuda.set_device(0)
+    solve_alone(make(0))
+    torch.cuda.synchronize()
+    service = []
+    for i in range(requests):
+        t0 = time.perf_counter()
+        solve_alone(make(i))
+        service.append(time.perf_counter() - t0)
+    capacity = requests / sum(service)
+    rate = load * capacity
+    arrivals = np.cumsum(np.random.Generator(np.random.PCG64(c["seed"] + 7)).exponential(1.0 / rate, requests))
+
+    def replay(n_slots):
+        with PMMServer(slots=n_slots) as srv:
+            t0 = time.perf_counter()
+            futs = []
+            for i, ta in enumerate(arrivals):
+                dt = t0 + ta - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                futs.append(srv.submit(make(i)))
+            res = [f.result() for f in futs]
+        return latency_summary(res), srv.peak_active
+
+    served, peak = replay(slots)
+    plain, _ = replay(1)
+    log("serving leg: %d requests at %.1fx plain capacity (%.2f req/s): p50 %.0f ms p99 %.0f ms %.2f req/s "
+        "(peak %d concurrent); plain path p50 %.0f ms p99 %.0f ms %.2f req/s" % (
+            requests, load, rate, served["p50_ms"], served["p99_ms"], served["throughput_req_per_s"], peak,
+            plain["p50_ms"], plain["p99_ms"], plain["throughput_req_per_s"]))
+    # CPU baseline: the C/OpenMP oracle solving the same requests in arrival
+    # order (same start, same EM schedule) until its time budget is spent
+    from oracle import pmm_ref
+
+    done, t_cpu = 0, 0.0
+    for i in sorted(range(requests), key=lambda j: probs[j][0].n_nodes * len(probs[j][1])):
+        t, q, cd, _ = probs[i]
+        rp = pmm_ref.RefProblem(t.c0, t.c1, t.parent, cd, q, None, 1.0 / (len(q) * cd.shape[0]))
+        t0 = time.perf_counter()
+        rp.em(np.full(t.n_nodes, 0.05), 0.5, 0.5, max_iter=c["em_iters"], tol=c["tol"])
+        t_cpu += time.perf_counter() - t0
+        done += 1
+        if t_cpu > cpu_budget_s:
+            break
+    threads = pmm_ref.threads()
+    return {
+        "requests": requests, "offered_load": load, "rate_req_per_s": round(rate, 3),
+        "p50_ms": round(served["p50_ms"], 2), "p99_ms": round(served["p99_ms"], 2),
+        "throughput_req_per_s": round(served["throughput_req_per_s"], 3), "slots": slots,
+        "peak_concurrent": peak,
+        "plain_path": {"p50_ms": round(plain["p50_ms"], 2), "p99_ms": round(plain["p99_ms"], 2),
+                       "throughput_req_per_s": round(plain["throughput_req_per_s"], 3),
+                       "capacity_req_per_s": round(capacity, 3)},
+        "workload": "leaves log-uniform %d-%d, sites uniform %d-%d, nu in [%g,%g], phi in [%g,%g], %d EM "
+                    "iterations (tol %g) from a uniform start, seed %d" % (
+                        c["n_lo"], c["n_hi"], c["K_lo"], c["K_hi"], c["nu"][0], c["nu"][1], c["phi"][0],
+                        c["phi"][1], c["em_iters"], c["tol"], c["seed"]),
+        "cpu_baseline": {"value": round(done / t_cpu, 3), "unit": "req/s", "cores": threads, "kind": "port",
+                         "sample": "C/OpenMP restatement of EM_solver (oracle/pmm_ref.c), %d threads, the %d "
+                                   "smallest of the %d requests solved one after another (%.2f s)"
+                                   % (threads, done, requests, t_cpu)},
+    }
+
+
+def scoring_leg(steps=10, warmup=2, sample=16):
+    """Batched scoring, bounded for the default run: the 768-tree workload
+    (512 NNI variants of a 2000-leaf tree + 256 trees of 100-5000 leaves,
+    200 sites) as one forest with its default launch plan, tree EM
+    iterations/s against one forest per tree on a sample."""
+    import torch
+
+    from laml_amd import sim
+    from laml_amd.forest import Forest, SiteTable
+
+    c = SCORING["nni768"]
+    problems = sim.scoring_workload(**c)
+    tables = {}
+    specs = []
+    for (t, q, cd) in problems:
+        if id(q) not in tables:
+            tables[id(q)] = SiteTable(q)
+        specs.append(sim.problem_spec(t, q, cd, sites=tables[id(q)]))
+    P = len(problems)
+    torch.cuda.set_device(0)
+
+    def start(F, probs):
+        d = torch.tensor(F.lengths_to_level([np.full(t.n_nodes, 0.05) for (t, _, _) in probs]),
+                         dtype=torch.float64, device=F.device)
+        nu = torch.full((len(probs),), 0.5, dtype=torch.float64, device=F.device)
+        phi = torch.full((len(probs),), 0.5, dtype=torch.float64, device=F.device)
+        return d, nu, phi
+
+    F = Forest(specs, device=0)
+    d, nu, phi = start(F, problems)
+    F.em(d, nu, phi, warmup, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nll, _ = F.em(d, nu, phi, steps, 0.0)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    nll_b = nll.cpu().numpy().copy()
+    F.close()
+    batch_rate = P * steps / el
+    nv = c["variants"]
+    idx = list(range(0, nv, nv // (sample // 2)))[:sample // 2] + \
+        list(range(nv, P, (P - nv) // (sample // 2)))[:sample // 2]
+    per_tree, worst = [], 0.0
+    for i in idx:
+        Fi = Forest([specs[i]], device=0)
+        di, nui, phii = start(Fi, [problems[i]])
+        Fi.em(di, nui, phii, warmup, 0.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nli, _ = Fi.em(di, nui, phii, steps, 0.0)
+        torch.cuda.synchronize()
+        per_tree.append((time.perf_counter() - t0) / steps)
+        got = float(nli[0].item())
+        worst = max(worst, abs(got - nll_b[i]) / abs(got))
+        Fi.close()
+    if worst > 1e-9:
+        raise SystemExit("batched scoring differs from one forest per tree (relative %g)" % worst)
+    k = len(idx) // 2
+    plain_s = nv * float(np.mean(per_tree[:k])) + (P - nv) * float(np.mean(per_tree[k:]))
+    plain_rate = P / plain_s
+    log("scoring leg: %d trees, %.3f ms per EM iteration of the batch, %.0f tree-iterations/s; one forest "
+        "per tree %.0f tree-iterations/s (%d sampled); x%.1f" % (P, el / steps * 1e3, batch_rate, plain_rate,
+                                                               len(idx), batch_rate / plain_rate))
+    # CPU baseline: the C/OpenMP oracle on sampled trees, the same EM steps
+    from oracle import pmm_ref
+
+    t_cpu, it_cpu = 0.0, 0
+    for i in idx[:4] + idx[k:k + 4]:
+        t, q, cd = problems[i]
+        rp = pmm_ref.RefProblem(t.c0, t.c1, t.parent, cd, q, None, 1.0 / (len(q) * cd.shape[0]))
+        t0 = time.perf_counter()
+        rp.em(np.full(t.n_nodes, 0.05), 0.5, 0.5, max_iter=3, tol=0.0, fixed_iters=3)
+        t_cpu += time.perf_counter() - t0
+        it_cpu += 3
+    threads = pmm_ref.threads()
+    return {
+        "trees": P, "steps": steps, "ms_per_step": round(el / steps * 1e3, 4),
+        "value": round(batch_rate, 2), "unit": "tree EM iters/s",
+        "one_forest_per_tree": {"tree_iters_per_s": round(plain_rate, 2), "sampled_trees": len(idx),
+                                "speedup_of_batch": round(batch_rate / plain_rate, 2),
+                                "max_rel_nll_diff": worst},
+        "launch_plan": "default",
+        "workload": "%d NNI variants of a %d-leaf Yule tree + %d Yule trees with log-uniform %d-%d leaves, %d "
+                    "sites, M in [%d,%d], nu=%g, phi=%g, EM from a uniform start" % (
+                        c["variants"], c["base_n"], c["others"], c["n_lo"], c["n_hi"], c["K"], c["m_lo"], c["m_hi"],
+                        c["nu"], c["phi"]),
+        "cpu_baseline": {"value": round(it_cpu / t_cpu, 3), "unit": "tree EM iters/s", "cores": threads,
+                         "kind": "port",
+                         "sample": "C/OpenMP restatement of EM_solver (oracle/pmm_ref.c), %d threads, 3 EM "
+                                   "iterations on each of 8 sampled trees (4 NNI variants, 4 independent "
+                                   "trees) in %.2f s" % (threads, t_cpu)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -530,6 +716,8 @@ def main():
                     help="write the batch forest's autotuned launch plan to this JSON file")
     ap.add_argument("--plan", default=None,
                     help="apply the launch plan in this JSON file (written by --save-plan) instead of autotuning")
+    ap.add_argument("--no-extra-legs", action="store_true",
+                    help="skip the serving-latency and batched-scoring legs of the default run")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the measurement, run 3 plain device copies of this many MiB (a known-size "
                          "reference for hardware byte counters)")
@@ -731,6 +919,9 @@ def main():
     else:
         res["cpu_baseline"] = None
     F.close()
+    if args.workload == "em10k" and not args.no_extra_legs:
+        res["serving"] = serving_leg()
+        res["batched_scoring"] = scoring_leg()
     if args.calib_copy_mib > 0:
         src = torch.ones(args.calib_copy_mib * (1 << 20) // 8, dtype=torch.float64, device=0)
         dst = torch.empty_like(src)

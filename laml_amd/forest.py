@@ -44,13 +44,17 @@ class TreeArrays:
         self.codes = codes  # int16 [n_leaves, K] rows in leaf post-order
         self.nodes = nodes
         self.poly = np.zeros(self.n_nodes, dtype=bool) if poly is None else np.asarray(poly, dtype=bool)
+        # height of every node (edges on its longest path down to a leaf):
+        # children precede parents, so sweeps of h[v] = 1 + max(h[children])
+        # settle one more level each, bottom up
         h = np.zeros(self.n_nodes, dtype=np.int64)
-        # post-order: children precede parents
-        for v in range(self.n_nodes):
-            a = self.c0[v]
-            if a >= 0:
-                b = self.c1[v]
-                h[v] = 1 + (h[a] if h[a] > h[b] else h[b])
+        inner = np.nonzero(~leaf)[0]
+        a, b = self.c0[inner], self.c1[inner]
+        while True:
+            hn = 1 + np.maximum(h[a], h[b])
+            if np.array_equal(hn, h[inner]):
+                break
+            h[inner] = hn
         self.height = h
 
 

@@ -199,9 +199,10 @@ def make_problem(n, K, m_lo, m_hi, alpha, nu, phi, seed, start_len=None):
     return tree, Q, codes, truth
 
 
-def problem_spec(tree, Q, codes, lengths=None, free=None, flags=3):
-    """ProblemSpec for the device forest straight from arrays."""
-    sites = SiteTable(Q)
+def problem_spec(tree, Q, codes, lengths=None, free=None, flags=3, sites=None):
+    """ProblemSpec for the device forest straight from arrays (``sites``: a
+    SiteTable of Q to share between problems with the same priors)."""
+    sites = SiteTable(Q) if sites is None else sites
     L = tree.length if lengths is None else lengths
     F = np.ones(tree.n_nodes, dtype=np.uint8) if free is None else free
     ta = TreeArrays(tree.c0, tree.c1, tree.parent, L, F, codes)
@@ -211,41 +212,44 @@ def problem_spec(tree, Q, codes, lengths=None, free=None, flags=3):
 
 
 def nni_variant(tree, codes, rng):
-    """The tree after one random NNI move (a child of a random internal
-    non-root node u swapped with u's sibling), as a new YuleTree in post-order
-    with the code rows permuted to its leaf order."""
-    T = tree.to_tree()
-    internal = [v for v in T.traverse_postorder() if v.children and v.parent is not None]
-    if not internal:
+    """The tree after one random NNI move: a child c of a random internal
+    non-root node u is swapped with u's sibling w.  Works on the arrays: the
+    nodes keep their indices, except that u moves to just before its parent
+    when w came after it (children must precede parents), so the leaves keep
+    their order and the code rows need no permutation.  Returns (YuleTree,
+    codes)."""
+    c0, c1, par = tree.c0.copy(), tree.c1.copy(), tree.parent.copy()
+    internal = np.nonzero((c0 >= 0) & (par >= 0))[0]
+    if len(internal) == 0:
         raise ValueError("tree has no internal non-root branch")
-    u = internal[int(rng.integers(len(internal)))]
-    v = u.parent
-    w = v.children[0] if v.children[1] is u else v.children[1]
-    c = u.children[int(rng.integers(2))]
-    u.children[u.children.index(c)] = w
-    w.parent = u
-    v.children[v.children.index(w)] = c
-    c.parent = v
-    nodes = list(T.traverse_postorder())
-    idx = {id(x): i for i, x in enumerate(nodes)}
-    N = len(nodes)
-    c0 = np.full(N, -1, dtype=np.int64)
-    c1 = np.full(N, -1, dtype=np.int64)
-    parent = np.full(N, -1, dtype=np.int64)
-    length = np.zeros(N)
-    names = []
-    for i, x in enumerate(nodes):
-        length[i] = x.edge_length
-        if x.children:
-            c0[i] = idx[id(x.children[0])]
-            c1[i] = idx[id(x.children[1])]
-        else:
-            names.append(x.label)
-        if x.parent is not None:
-            parent[i] = idx[id(x.parent)]
-    row = {name: i for i, name in enumerate(tree.leaf_names)}
-    perm = np.asarray([row[nm] for nm in names], dtype=np.int64)
-    return YuleTree(c0, c1, parent, length, names), codes[perm]
+    u = int(internal[int(rng.integers(len(internal)))])
+    v = int(par[u])
+    w = int(c1[v] if c0[v] == u else c0[v])
+    if int(rng.integers(2)) == 0:
+        c, c0[u] = int(c0[u]), w
+    else:
+        c, c1[u] = int(c1[u]), w
+    if c0[v] == w:
+        c0[v] = c
+    else:
+        c1[v] = c
+    par[w] = u
+    par[c] = v
+    length = tree.length
+    if w > u:
+        # order: ... u-1, u+1 .. v-1, u, v ...
+        N = len(c0)
+        order = np.concatenate([np.arange(u), np.arange(u + 1, v), [u], np.arange(v, N)])
+        new = np.empty(N, dtype=np.int64)
+        new[order] = np.arange(N)
+
+        def remap(a):
+            a = a[order]
+            return np.where(a >= 0, new[np.maximum(a, 0)], -1)
+
+        c0, c1, par = remap(c0), remap(c1), remap(par)
+        length = length[order]
+    return YuleTree(c0, c1, par, length.copy(), list(tree.leaf_names)), codes
 
 
 def scoring_workload(base_n, variants, others, n_lo, n_hi, K, m_lo, m_hi, alpha, nu, phi, seed):
