@@ -19,6 +19,14 @@ the exact EM M-step, stopped by the same relative criterion (1e-5,
 fast_em/laml.py:46,183).  The returned parameters are those of the returned
 likelihood (the reference returns the previous iterate's with the last
 likelihood).
+
+``EMOptimizer(..., m_step="surrogate")`` runs the reference's own EM instead
+(fast_em/laml.py:112-194) in FP64 without the EPS clamps: device E-steps, and
+an M-step that minimises the reference's joint surrogate (M_step_loss_fn,
+fast_em/laml.py:62-78: the expected complete-data log-likelihood over log
+branch lengths and logit nu, phi, with the raw expected counts) by 100
+L-BFGS iterations on the host, its stopping rule, and its return values
+(the parameters before the last M-step, with the likelihood after it).
 """
 import random as _random
 
@@ -36,6 +44,8 @@ EM_STOPPING_CRITERION = 1e-5
 FASTEM_DMIN = EPS
 FASTEM_DMAX = 1e3
 MAX_EM_ITER = 10000
+# L-BFGS iterations of one surrogate M-step (fast_em/laml.py:45)
+M_STEP_DESCENT_STEPS = 100
 
 try:  # the reference hands networkx graphs around; plain arrays work without it
     import networkx as _nx
@@ -202,16 +212,76 @@ def fastem_problem(children, root, branch_lengths, branch_mask, character_matrix
     return ProblemSpec([ta], sites, nobs, phimin, flags), post
 
 
-class EMOptimizer:
-    """fast_em.laml.EMOptimizer (fast_em/laml.py:196-276) on the device."""
+def _log1mexp(x):
+    """log(1 - e^-x) for x > 0 (fast_em/laml.py:54-57)."""
+    return np.where(x > np.log(2.0), np.log1p(-np.exp(-np.minimum(x, 1e300))), np.log(-np.expm1(-x)))
 
-    def __init__(self, mutation_priors, character_matrix, verbose=False):
+
+def surrogate_loss(x, S, n_missing, n_not_missing, rt):
+    """The reference's M-step loss (M_step_loss_fn, fast_em/laml.py:62-78)
+    and its gradient, in FP64 without clamps.  x = (log branch lengths [N],
+    logit nu, logit phi); S: raw expected counts S0..S4 per branch [N, 5]
+    (summed over sites); rt: sum over missing leaf entries of P(not silenced
+    | D), i.e. num_missing minus the summed leaf responsibilities."""
+    N = S.shape[0]
+    d = np.exp(x[:N])
+    nu = 1.0 / (1.0 + np.exp(-x[N]))
+    phi = 1.0 / (1.0 + np.exp(-x[N + 1]))
+    dn = d * nu
+    l1 = _log1mexp(d)
+    l2 = _log1mexp(dn)
+    B = S[:, 2] + S[:, 4]
+    q = (-S[:, 0] * d * (1.0 + nu) + S[:, 1] * (l1 - d * nu) + B * l2 - S[:, 3] * d * nu)
+    c6 = n_not_missing * np.log1p(-phi) + rt * np.log(phi)
+    loss = -(q.sum() + c6)
+    # derivatives of -loss
+    gd = -S[:, 0] * (1.0 + nu) + S[:, 1] * (1.0 / np.expm1(d) - nu) + B * nu / np.expm1(dn) - S[:, 3] * nu
+    gnu = np.sum(d * (-S[:, 0] - S[:, 1] - S[:, 3] + B / np.expm1(dn)))
+    gphi = -n_not_missing / (1.0 - phi) + rt / phi
+    g = np.empty(N + 2)
+    g[:N] = -gd * d
+    g[N] = -gnu * nu * (1.0 - nu)
+    g[N + 1] = -gphi * phi * (1.0 - phi)
+    return loss, g
+
+
+def surrogate_mstep(x0, free, S, n_missing, n_not_missing, rt, steps=M_STEP_DESCENT_STEPS):
+    """One surrogate M-step (fast_em/laml.py:101-110): `steps` L-BFGS
+    iterations over the free entries of x (the reference masks the gradient
+    of the fixed ones, so they keep their values)."""
+    from scipy.optimize import minimize
+
+    idx = np.nonzero(free)[0]
+
+    def f(z):
+        x = x0.copy()
+        x[idx] = z
+        loss, g = surrogate_loss(x, S, n_missing, n_not_missing, rt)
+        return loss, g[idx]
+
+    res = minimize(f, x0[idx], jac=True, method="L-BFGS-B",
+                   options={"maxiter": steps, "maxcor": 10, "ftol": 0.0, "gtol": 0.0})
+    x = x0.copy()
+    x[idx] = res.x
+    return x
+
+
+class EMOptimizer:
+    """fast_em.laml.EMOptimizer (fast_em/laml.py:196-276) on the device.
+    m_step: "exact" (the default) solves each M-step exactly with the
+    device's EM M-step; "surrogate" runs the reference's joint L-BFGS
+    surrogate M-step (see surrogate_mstep)."""
+
+    def __init__(self, mutation_priors, character_matrix, verbose=False, m_step="exact"):
+        if m_step not in ("exact", "surrogate"):
+            raise ValueError("m_step must be 'exact' or 'surrogate'")
         self.mutation_priors = np.asarray(mutation_priors, dtype=np.float64)
         self.character_matrix = np.asarray(character_matrix)
         self.num_leaves = self.character_matrix.shape[0]
         self.alphabet_size = self.mutation_priors.shape[1]
         self.num_characters = self.character_matrix.shape[1]
         self.verbose = verbose
+        self.m_step = m_step
 
     def optimize(self, tree, branch_lengths, nu, phi, branch_mask, fixed_nu=False, fixed_phi=False):
         if isinstance(tree, dict) and "children" in tree:
@@ -230,6 +300,13 @@ class EMOptimizer:
         if not torch.cuda.is_available():
             raise RuntimeError("EMOptimizer needs a ROCm GPU")
         F = Forest([spec], device=torch.cuda.current_device(), dmin=FASTEM_DMIN, dmax=FASTEM_DMAX)
+        if self.m_step == "surrogate":
+            try:
+                return self._optimize_surrogate(F, spec, post, np.asarray(branch_lengths, dtype=np.float64),
+                                                np.asarray(branch_mask, dtype=bool), nu, phi, fixed_nu, fixed_phi,
+                                                len(children))
+            finally:
+                F.close()
         try:
             d = torch.tensor(F.lengths_to_level([spec.trees[0].length]), dtype=torch.float64, device=F.device)
             clamp = lambda x: min(max(float(x), EPS), 1.0 - EPS)
@@ -245,3 +322,57 @@ class EMOptimizer:
         finally:
             F.close()
         return res
+
+    def _optimize_surrogate(self, F, spec, post, branch_lengths, branch_mask, nu, phi, fixed_nu, fixed_phi, n):
+        """optimize_parameters_expectation_maximization (fast_em/laml.py:112-194)
+        in FP64: device E-steps and likelihoods, host surrogate M-steps."""
+        N = spec.trees[0].n_nodes
+        dev = F.device
+        # the start: lengths as given (positive), nu and phi inside (0, 1)
+        # (fast_em/laml.py:128-132 clamp them at EPS; FP64 needs no clamp
+        # beyond keeping the logit finite)
+        lp = np.asarray(spec.trees[0].length, dtype=np.float64)  # post-order of the device tree
+        if np.any(lp <= 0.0):
+            raise ValueError("surrogate M-step needs positive branch lengths")
+        m = np.asarray([min(max(float(nu), 1e-300), 1.0 - 1e-16), min(max(float(phi), 1e-300), 1.0 - 1e-16)])
+        x = np.concatenate([np.log(lp), np.log(m / (1.0 - m))])
+        free = np.concatenate([spec.trees[0].free.astype(bool), [not fixed_nu, not fixed_phi]])
+        C = self.character_matrix
+        n_missing = float((C == -1).sum())
+        n_not_missing = float((C != -1).sum())
+
+        def to_dev(xv):
+            d = torch.tensor(F.lengths_to_level([np.exp(xv[:N])]), dtype=torch.float64, device=dev)
+            mp = 1.0 / (1.0 + np.exp(-xv[N:]))
+            return (d, torch.tensor([mp[0]], dtype=torch.float64, device=dev),
+                    torch.tensor([mp[1]], dtype=torch.float64, device=dev))
+
+        def nllh(xv):
+            return float(F.nll(*to_dev(xv)).item())
+
+        previous = nllh(x)
+        iteration = 0
+        while True:
+            iteration += 1
+            counts, rt, _ = F.estep(*to_dev(x))
+            S = np.empty((N, 5))
+            S[F.order] = counts.view(-1, 5).cpu().numpy()
+            x_new = surrogate_mstep(x, free, S, n_missing, n_not_missing, float(rt.item()))
+            current = nllh(x_new)
+            if abs(current - previous) / abs(previous) < EM_STOPPING_CRITERION:
+                break
+            if np.isnan(current):
+                current = previous
+                break
+            x = x_new
+            previous = current
+        # the reference returns the parameters before the last M-step with the
+        # likelihood after it (fast_em/laml.py:181-194)
+        out_bl = np.empty(n)
+        for i, v in enumerate(post):
+            out_bl[v] = np.exp(x[i])
+        mp = 1.0 / (1.0 + np.exp(-x[N:]))
+        self.last_x = x
+        self.last_x_new = x_new
+        return {"nllh": current, "branch_lengths": out_bl, "nu": float(mp[0]), "phi": float(mp[1]),
+                "em_iterations": iteration}

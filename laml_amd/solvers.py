@@ -683,14 +683,28 @@ class fastEM_solver(EM_solver):
     ``parse_data``; ``score_tree`` runs the fastEM model through
     ``laml_amd.fastem.EMOptimizer`` (exact FP64 EM on the device) with the
     reference's start values and always-optimised nu and phi, and the exact
-    reference EM for 'final_optimization' (fastEM_solver.py:290-292)."""
+    reference EM for 'final_optimization' (fastEM_solver.py:290-292).
+
+    Options (keywords, not in the reference's constructor):
+    m_step -- "exact" (default) or "surrogate": the reference's joint L-BFGS
+    surrogate M-step (fast_em/laml.py:62-110), see EMOptimizer;
+    local_mask -- which branches a local-scoring strategy (``fixed_brlen``
+    with ``nodes_to_recompute``) re-optimises: "named" (default) the branches
+    its anchor pairs name; "reference" none, as the reference's comparison
+    of node labels with node objects (fastEM_solver.py:307-316) leaves every
+    branch fixed."""
     solver_name = "fastEM_solver"
 
-    def __init__(self, treeList, data, prior, params={'nu': 0, 'phi': 0, 'sigma': 0}):
+    def __init__(self, treeList, data, prior, params={'nu': 0, 'phi': 0, 'sigma': 0}, *, m_step="exact",
+                 local_mask="named"):
         from .fastem import EMOptimizer, parse_data, parse_tree
 
+        if local_mask not in ("named", "reference"):
+            raise ValueError("local_mask must be 'named' or 'reference'")
         super().__init__(treeList, data, prior, params={'nu': 0, 'phi': 0, 'sigma': 0})
         self.solver_name = "fastEM_solver"
+        self.m_step = m_step
+        self.local_mask = local_mask
         self.data = data
         self.prior = prior
         self.myEMOptimizers = []
@@ -707,7 +721,7 @@ class fastEM_solver(EM_solver):
                 C, Qr = phy.character_matrix, phy.mutation_priors
             self.ordered_leaf_labels.append(labels)
             parse_tree(tree, has_branch_mask=False, ordered_leaf_labels=labels)
-            self.myEMOptimizers.append(EMOptimizer(Qr, C))
+            self.myEMOptimizers.append(EMOptimizer(Qr, C, m_step=m_step))
 
     def score_tree(self, strategy={'ultra_constr': False, 'fixed_phi': None, 'fixed_nu': None, 'fixed_brlen': None,
                                    'nodes_to_recompute': None}):
@@ -715,7 +729,8 @@ class fastEM_solver(EM_solver):
         by ``nodes_to_recompute`` (anchor pairs) with positive length are
         re-optimised, all others stay fixed when ``fixed_brlen`` is given
         (the reference compares node labels to node objects there, so it
-        re-optimises none; this re-optimises the ones it names)."""
+        re-optimises none; this re-optimises the ones it names unless the
+        solver was built with local_mask="reference")."""
         from .fastem import parse_tree
 
         if 'final_optimization' in strategy:
@@ -727,7 +742,7 @@ class fastEM_solver(EM_solver):
         results = []
         for tidx, tree in enumerate(self.trees):
             recompute = set()
-            if nodes_to_recompute:
+            if nodes_to_recompute and self.local_mask == "named":
                 recompute = {id(v) for v in find_LCAs(tree, list(nodes_to_recompute[tidx]))}
             for node in tree.traverse_postorder():
                 node.mark_fixed = True
