@@ -68,13 +68,39 @@ class PipelinedBatch:
         return plans
 
     def em(self, params, max_iter, tol):
-        """EM on every part, each enqueued on its stream (the parts run
-        concurrently); returns per part (nll, iters) device tensors, valid
-        after synchronize()."""
-        out = []
-        for F, s, prm in zip(self.forests, self.streams, params):
-            with torch.cuda.stream(s):
-                out.append(F.em(*prm, max_iter, tol))
+        """EM on every part, each on its stream (the parts run concurrently);
+        returns per part (nll, iters) device tensors, valid after
+        synchronize().  With a convergence test (tol > 0) a part's EM loop
+        waits on its stream every few steps to read the problems' done
+        flags, so each part is driven from a host thread of its own; with a
+        fixed number of steps (tol <= 0) the calls only enqueue work and run
+        on the calling thread."""
+        if tol <= 0 or len(self.forests) == 1:
+            out = []
+            for F, s, prm in zip(self.forests, self.streams, params):
+                with torch.cuda.stream(s):
+                    out.append(F.em(*prm, max_iter, tol))
+            return out
+        import threading
+
+        out = [None] * len(self.forests)
+        errors = []
+
+        def run(j):
+            try:
+                torch.cuda.set_device(self.device)
+                with torch.cuda.stream(self.streams[j]):
+                    out[j] = self.forests[j].em(*params[j], max_iter, tol)
+            except BaseException as e:  # re-raised on the calling thread
+                errors.append(e)
+
+        threads = [threading.Thread(target=run, args=(j,)) for j in range(len(self.forests))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
         return out
 
     def synchronize(self):

@@ -276,33 +276,57 @@ def score_batch(jobs, data, prior, params, solver_cls=EM_solver, max_iter=1000, 
     them together with a per-problem convergence test (EM_solver.py:461-512).
     Returns a list of (solver, score, status).  Candidates with the
     ultrametric constraint, and every candidate of a solver other than
-    EM_solver (whose score_tree is not this EM), are scored one by one."""
+    EM_solver (whose score_tree is not this EM), are scored one by one.
+
+    The global random generator is left as it was: the reference scores a
+    batch in forked pool workers (Topology_search_parallel.py:25-26), so
+    each candidate draws its seed from a copy of the caller's generator and
+    the caller's own draws (move shuffles, annealed acceptance) do not
+    depend on the scoring.  Each candidate here starts from the caller's
+    state, as a candidate alone in a fresh worker does."""
     out = [None] * len(jobs)
     batch = []
     batchable = getattr(solver_cls, "solver_name", None) == "EM_solver"
-    for j, (nwks, st) in enumerate(jobs):
-        s = solver_cls(nwks, data, prior, params)
-        if st.get('ultra_constr') or not batchable:
-            sc, status = s.score_tree(strategy=st)
-            out[j] = (s, sc, status)
+    caller_state = _random.getstate()
+    try:
+        for j, (nwks, st) in enumerate(jobs):
+            _random.setstate(caller_state)
+            _prepare_candidate(j, nwks, st, data, prior, params, solver_cls, batchable, out, batch)
+    finally:
+        _random.setstate(caller_state)
+    _score_prepared(batch, out, max_iter, tol)
+    return out
+
+
+def _prepare_candidate(j, nwks, st, data, prior, params, solver_cls, batchable, out, batch):
+    """score_batch's set-up of candidate j: scored at once if it is not
+    batchable, else its start drawn and queued in `batch`."""
+    s = solver_cls(nwks, data, prior, params)
+    if st.get('ultra_constr') or not batchable:
+        sc, status = s.score_tree(strategy=st)
+        out[j] = (s, sc, status)
+        return
+    fixed_phi, fixed_nu, fixed_brlen = st['fixed_phi'], st['fixed_nu'], st['fixed_brlen']
+    randseed = int(_random.random() * 10000)
+    s.em_seed = randseed
+    for t, tree in enumerate(s.trees):
+        for x in tree.traverse_postorder():
+            x.mark_fixed = False
+        if fixed_brlen is None:
             continue
-        fixed_phi, fixed_nu, fixed_brlen = st['fixed_phi'], st['fixed_nu'], st['fixed_brlen']
-        randseed = int(_random.random() * 10000)
-        s.em_seed = randseed
-        for t, tree in enumerate(s.trees):
-            for x in tree.traverse_postorder():
-                x.mark_fixed = False
-            if fixed_brlen is None:
-                continue
-            nodes = find_LCAs(tree, list(fixed_brlen[t].keys()))
-            for i, (a, b) in enumerate(fixed_brlen[t]):
-                nodes[i].edge_length = fixed_brlen[t][(a, b)]
-                nodes[i].mark_fixed = True
-        _random.seed(a=randseed)
-        x0 = s.ini_all(fixed_phi=fixed_phi, fixed_nu=fixed_nu)
-        s.x2params(x0, fixed_phi=fixed_phi, fixed_nu=fixed_nu)
-        flags = (LAML_OPT_PHI if fixed_phi is None else 0) | (LAML_OPT_NU if fixed_nu is None else 0)
-        batch.append((j, s, flags))
+        nodes = find_LCAs(tree, list(fixed_brlen[t].keys()))
+        for i, (a, b) in enumerate(fixed_brlen[t]):
+            nodes[i].edge_length = fixed_brlen[t][(a, b)]
+            nodes[i].mark_fixed = True
+    _random.seed(a=randseed)
+    x0 = s.ini_all(fixed_phi=fixed_phi, fixed_nu=fixed_nu)
+    s.x2params(x0, fixed_phi=fixed_phi, fixed_nu=fixed_nu)
+    flags = (LAML_OPT_PHI if fixed_phi is None else 0) | (LAML_OPT_NU if fixed_nu is None else 0)
+    batch.append((j, s, flags))
+
+
+def _score_prepared(batch, out, max_iter, tol):
+    """EM on the queued candidates as one device forest."""
     if batch:
         specs = [s._problem_spec(flags) for (_, s, flags) in batch]
         dev = torch.cuda.current_device()
