@@ -220,7 +220,8 @@ SCORING = {
 SERVING = {
     # Poisson stream of likelihood + EM requests of mixed sizes
     "serve": dict(requests=48, n_lo=200, n_hi=50000, K_lo=20, K_hi=500, nu=(0.02, 0.2), phi=(0.05, 0.3),
-                  em_iters=50, tol=1e-8, slots=4, loads=(0.5, 1.0, 2.0), seed=20261209),
+                  em_iters=50, tol=1e-8, slots=4, loads=(0.5, 1.0, 2.0), seed=20261209,
+                  batch_bytes=512 << 20, batch_max=8),
 }
 
 
@@ -265,10 +266,10 @@ def bench_serving(args):
     capacity = len(reqs) / sum(service)
     log("plain path: mean service %.1f ms, capacity %.2f req/s" % (1e3 * np.mean(service), capacity))
 
-    def replay(slots, rate, seed):
+    def replay(slots, rate, seed, batch_bytes=0):
         arr = np.cumsum(np.random.Generator(np.random.PCG64(seed)).exponential(1.0 / rate, len(reqs)))
         results = [None] * len(reqs)
-        with PMMServer(slots=slots) as srv:
+        with PMMServer(slots=slots, batch_bytes=batch_bytes, batch_max=c["batch_max"]) as srv:
             t0 = time.perf_counter()
             futs = []
             for i, ta in enumerate(arr):
@@ -278,20 +279,25 @@ def bench_serving(args):
                 futs.append(srv.submit(make(i)))
             for i, f in enumerate(futs):
                 results[i] = f.result()
-        return latency_summary(results), srv.peak_active
+        return latency_summary(results), srv.peak_active, srv.batches
 
     runs = []
     for load in c["loads"]:
         rate = load * capacity
-        served, peak = replay(c["slots"], rate, c["seed"] + 7)
-        plain, _ = replay(1, rate, c["seed"] + 7)
+        served, peak, batches = replay(c["slots"], rate, c["seed"] + 7, c["batch_bytes"])
+        unbatched, _, _ = replay(c["slots"], rate, c["seed"] + 7)
+        plain, _, _ = replay(1, rate, c["seed"] + 7)
         log("offered load %.1fx plain capacity (%.2f req/s): served p50 %.0f ms p99 %.0f ms %.2f req/s "
-            "(peak %d concurrent); plain p50 %.0f ms p99 %.0f ms %.2f req/s" % (
+            "(peak %d concurrent, %d solves for %d requests); unbatched p50 %.0f ms p99 %.0f ms; plain p50 %.0f "
+            "ms p99 %.0f ms %.2f req/s" % (
                 load, rate, served["p50_ms"], served["p99_ms"], served["throughput_req_per_s"], peak,
-                plain["p50_ms"], plain["p99_ms"], plain["throughput_req_per_s"]))
+                len(batches), sum(batches), unbatched["p50_ms"], unbatched["p99_ms"], plain["p50_ms"],
+                plain["p99_ms"], plain["throughput_req_per_s"]))
         runs.append({"offered_load": load, "rate_req_per_s": round(rate, 3),
                      "served": {k: round(v, 3) for k, v in served.items()},
-                     "plain": {k: round(v, 3) for k, v in plain.items()}, "peak_concurrent": peak})
+                     "served_unbatched": {k: round(v, 3) for k, v in unbatched.items()},
+                     "plain": {k: round(v, 3) for k, v in plain.items()}, "peak_concurrent": peak,
+                     "device_solves": len(batches)})
     top = runs[-1]
     res = {
         "metric": "p99 solve latency at %.1fx the plain path's capacity" % top["offered_load"],
@@ -308,9 +314,11 @@ def bench_serving(args):
         "data": "synthetic (seeded Yule trees, PMM-simulated characters, Poisson arrivals)",
         "config": {"workload": "%d requests, leaves log-uniform %d-%d, sites uniform %d-%d, nu in [%g,%g], "
                                "phi in [%g,%g], %d EM iterations (tol %g) from a uniform start; %d slots, "
-                               "admission by device footprint" % (
+                               "admission by device footprint, requests of at most %d MiB batched up to %d "
+                               "MiB and %d requests" % (
                                    c["requests"], c["n_lo"], c["n_hi"], c["K_lo"], c["K_hi"], c["nu"][0],
-                                   c["nu"][1], c["phi"][0], c["phi"][1], c["em_iters"], c["tol"], c["slots"]),
+                                   c["nu"][1], c["phi"][0], c["phi"][1], c["em_iters"], c["tol"], c["slots"],
+                                   c["batch_bytes"] >> 22, c["batch_bytes"] >> 20, c["batch_max"]),
                    "plain_capacity_req_per_s": round(capacity, 3)},
         "runs": runs,
         "cpu_baseline": None,
@@ -518,7 +526,8 @@ def serving_leg(requests=24, load=2.0, slots=4, cpu_budget_s=6.0):
     workload's ranges (leaves log-uniform 200-50,000, sites 20-500, nu in
     [0.02,0.2], phi in [0.05,0.3], 50 EM iterations, tol 1e-8) offered at
     `load` x the plain path's capacity, served by the continuous-batching
-    server (`slots` concurrent solves, admission by device footprint) and by
+    server (`slots` concurrent solves, admission by device footprint, small
+    requests sharing a forest), by the same server without batching and by
     the plain per-request path (one solve at a time) on the same arrivals."""
     import torch
 
@@ -552,8 +561,8 @@ def serving_leg(requests=24, load=2.0, slots=4, cpu_budget_s=6.0):
     rate = load * capacity
     arrivals = np.cumsum(np.random.Generator(np.random.PCG64(c["seed"] + 7)).exponential(1.0 / rate, requests))
 
-    def replay(n_slots):
-        with PMMServer(slots=n_slots) as srv:
+    def replay(n_slots, batch_bytes=0):
+        with PMMServer(slots=n_slots, batch_bytes=batch_bytes, batch_max=c["batch_max"]) as srv:
             t0 = time.perf_counter()
             futs = []
             for i, ta in enumerate(arrivals):
@@ -562,13 +571,16 @@ def serving_leg(requests=24, load=2.0, slots=4, cpu_budget_s=6.0):
                     time.sleep(dt)
                 futs.append(srv.submit(make(i)))
             res = [f.result() for f in futs]
-        return latency_summary(res), srv.peak_active
+        return latency_summary(res), srv.peak_active, srv.batches
 
-    served, peak = replay(slots)
-    plain, _ = replay(1)
+    served, peak, batches = replay(slots, c["batch_bytes"])
+    unbatched, _, _ = replay(slots)
+    plain, _, _ = replay(1)
     log("serving leg: %d requests at %.1fx plain capacity (%.2f req/s): p50 %.0f ms p99 %.0f ms %.2f req/s "
-        "(peak %d concurrent); plain path p50 %.0f ms p99 %.0f ms %.2f req/s" % (
+        "(peak %d concurrent, %d device solves); unbatched p50 %.0f ms p99 %.0f ms %.2f req/s; plain path p50 "
+        "%.0f ms p99 %.0f ms %.2f req/s" % (
             requests, load, rate, served["p50_ms"], served["p99_ms"], served["throughput_req_per_s"], peak,
+            len(batches), unbatched["p50_ms"], unbatched["p99_ms"], unbatched["throughput_req_per_s"],
             plain["p50_ms"], plain["p99_ms"], plain["throughput_req_per_s"]))
     # CPU baseline: the C/OpenMP oracle solving the same requests in arrival
     # order (same start, same EM schedule) until its time budget is spent
@@ -589,7 +601,11 @@ def serving_leg(requests=24, load=2.0, slots=4, cpu_budget_s=6.0):
         "requests": requests, "offered_load": load, "rate_req_per_s": round(rate, 3),
         "p50_ms": round(served["p50_ms"], 2), "p99_ms": round(served["p99_ms"], 2),
         "throughput_req_per_s": round(served["throughput_req_per_s"], 3), "slots": slots,
-        "peak_concurrent": peak,
+        "peak_concurrent": peak, "device_solves": len(batches),
+        "batching": "requests of at most %d MiB share a forest, up to %d MiB and %d requests" % (
+            c["batch_bytes"] >> 22, c["batch_bytes"] >> 20, c["batch_max"]),
+        "unbatched": {"p50_ms": round(unbatched["p50_ms"], 2), "p99_ms": round(unbatched["p99_ms"], 2),
+                      "throughput_req_per_s": round(unbatched["throughput_req_per_s"], 3)},
         "plain_path": {"p50_ms": round(plain["p50_ms"], 2), "p99_ms": round(plain["p99_ms"], 2),
                        "throughput_req_per_s": round(plain["throughput_req_per_s"], 3),
                        "capacity_req_per_s": round(capacity, 3)},

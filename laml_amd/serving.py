@@ -14,6 +14,16 @@ while others run starts as soon as a slot and memory free up.
 Each request runs the same code path as a solve on its own (its own forest,
 the same launches, on its own stream), so its result does not depend on what
 else is being served; the tests check that bit for bit.
+
+With batching on (``batch_bytes`` > 0), a slot whose next request is small
+(footprint at most ``batch_bytes`` / 4) also takes the other small queued
+requests with the same EM budget, in queue order, up to ``batch_bytes`` and
+``batch_max`` requests, and solves them as one device-resident forest: one
+problem per request, one launch sequence for all of them, each problem
+stopping at its own convergence test.  Small requests then share launches
+instead of each paying a full launch sequence; each result equals the
+request solved alone to within the engine's tolerance (the launch shapes of
+a shared forest change the order of floating-point sums over sites).
 """
 import collections
 import threading
@@ -106,16 +116,53 @@ def solve_alone(req, device=0):
     return out
 
 
+def solve_batch(reqs, device=0):
+    """Solve requests with the same EM budget as one forest (a problem per
+    request) on the calling thread's current stream; a result tuple per
+    request as ``solve_alone`` returns it."""
+    if len(reqs) == 1:
+        return [solve_alone(reqs[0], device)]
+    max_iter, tol = reqs[0].max_iter, reqs[0].tol
+    if any(r.max_iter != max_iter or r.tol != tol for r in reqs):
+        raise ValueError("a batch shares one EM budget")
+    F = Forest([r.spec for r in reqs], device=device)
+    try:
+        d = torch.tensor(F.lengths_to_level([x for r in reqs for x in r.lengths]), dtype=torch.float64,
+                         device=F.device)
+        nu = torch.tensor([r.nu for r in reqs], dtype=torch.float64, device=F.device)
+        phi = torch.tensor([r.phi for r in reqs], dtype=torch.float64, device=F.device)
+        nll, iters = F.em(d, nu, phi, max_iter, tol)
+        nll_h, it_h = nll.cpu().numpy(), iters.cpu().numpy()
+        nu_h, phi_h = nu.cpu().numpy(), phi.cpu().numpy()
+        per_tree = F.level_to_tree_lengths(d.cpu().numpy())
+    finally:
+        F.close()
+    out, t = [], 0
+    for i, r in enumerate(reqs):
+        nt = len(r.spec.trees)
+        out.append((float(nll_h[i]), per_tree[t:t + nt], float(nu_h[i]), float(phi_h[i]), int(it_h[i])))
+        t += nt
+    return out
+
+
 class PMMServer:
     """Continuous-batching server: ``slots`` concurrent solves, admission by
-    estimated device footprint within ``hbm_budget`` bytes.
+    estimated device footprint within ``hbm_budget`` bytes, and (with
+    ``batch_bytes`` > 0) small requests solved together as one forest.
 
     ``runner(req)`` computes a request's result tuple (nll, lengths, nu, phi,
-    iters); the default runs ``solve_alone`` on the slot's stream."""
+    iters); the default runs ``solve_alone`` on the slot's stream.
+    ``batch_runner(reqs)`` computes the tuples of a batch; the default runs
+    ``solve_batch`` on the slot's stream."""
 
-    def __init__(self, device=0, slots=4, hbm_budget=None, runner=None, use_streams=True):
+    def __init__(self, device=0, slots=4, hbm_budget=None, runner=None, use_streams=True, batch_bytes=0,
+                 batch_max=16, batch_runner=None):
         self.device = device
         self.slots = int(slots)
+        self.batch_bytes = int(batch_bytes)
+        self.batch_max = max(1, int(batch_max))
+        self._batch_runner = batch_runner
+        self.batches = []  # sizes of the batches solved
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(device)
             hbm_budget = int(free * 0.8)
@@ -125,7 +172,7 @@ class PMMServer:
         self.active = 0
         self.peak_active = 0
         self._runner = runner
-        self._use_streams = use_streams and runner is None
+        self._use_streams = use_streams and runner is None and batch_runner is None
         self._queue = collections.deque()
         self._cv = threading.Condition()
         self._closing = False
@@ -148,18 +195,37 @@ class PMMServer:
             self._cv.notify_all()
         return fut
 
+    def _small(self, req):
+        return self.batch_bytes > 0 and req.bytes * 4 <= self.batch_bytes
+
     def _admit(self):
-        # FIFO: the head request runs as soon as a slot and its memory are free
+        # FIFO: the head request runs as soon as a slot and its memory are
+        # free; a small head takes the other small queued requests with its
+        # EM budget along (in queue order), within the batch limits
         while True:
             if self._closing and not self._queue:
                 return None
             if self._queue and self.in_use + self._queue[0][0].bytes <= self.budget:
                 req, fut = self._queue.popleft()
-                self.in_use += req.bytes
-                self.active += 1
+                items = [(req, fut)]
+                total = req.bytes
+                if self._small(req):
+                    keep = collections.deque()
+                    while self._queue:
+                        r2, f2 = self._queue.popleft()
+                        if (len(items) < self.batch_max and self._small(r2) and r2.max_iter == req.max_iter
+                                and r2.tol == req.tol and total + r2.bytes <= self.batch_bytes
+                                and self.in_use + total + r2.bytes <= self.budget):
+                            items.append((r2, f2))
+                            total += r2.bytes
+                        else:
+                            keep.append((r2, f2))
+                    self._queue = keep
+                self.in_use += total
+                self.active += len(items)
                 self.peak_in_use = max(self.peak_in_use, self.in_use)
                 self.peak_active = max(self.peak_active, self.active)
-                return req, fut
+                return items
             self._cv.wait()
 
     def _work(self):
@@ -169,26 +235,38 @@ class PMMServer:
             stream = torch.cuda.Stream(device=self.device)
         while True:
             with self._cv:
-                item = self._admit()
-            if item is None:
+                items = self._admit()
+            if items is None:
                 return
-            req, fut = item
+            reqs = [r for r, _ in items]
             t_admit = time.perf_counter()
             try:
-                if stream is not None:
+                if len(reqs) > 1:
+                    if self._batch_runner is not None:
+                        outs = self._batch_runner(reqs)
+                    elif stream is not None:
+                        with torch.cuda.stream(stream):
+                            outs = solve_batch(reqs, self.device)
+                    else:
+                        outs = [self._runner(r) for r in reqs]
+                elif stream is not None:
                     with torch.cuda.stream(stream):
-                        out = solve_alone(req, self.device)
+                        outs = [solve_alone(reqs[0], self.device)]
                 else:
-                    out = self._runner(req)
-                res = SolveResult(req.rid, out[0], out[1], out[2], out[3], out[4], req.t_arrive, t_admit,
-                                  time.perf_counter())
-                fut.set_result(res)
-            except BaseException as e:  # the request fails, the server goes on
-                fut.set_exception(e)
+                    outs = [self._runner(reqs[0])]
+                t_done = time.perf_counter()
+                for (req, fut), out in zip(items, outs):
+                    fut.set_result(SolveResult(req.rid, out[0], out[1], out[2], out[3], out[4], req.t_arrive,
+                                               t_admit, t_done))
+            except BaseException as e:  # the requests fail, the server goes on
+                for _, fut in items:
+                    if not fut.done():
+                        fut.set_exception(e)
             finally:
                 with self._cv:
-                    self.in_use -= req.bytes
-                    self.active -= 1
+                    self.batches.append(len(items))
+                    self.in_use -= sum(r.bytes for r in reqs)
+                    self.active -= len(items)
                     self._cv.notify_all()
 
     def close(self):

@@ -1,12 +1,16 @@
 """Serving loop on the GPU: every request's result is the same, bit for bit,
 whether it is solved alone or in a mix of concurrent requests of different
-sizes admitted under a memory budget that forces queueing."""
+sizes admitted under a memory budget that forces queueing.  Requests solved
+together as one forest (batching of small requests) equal their solves
+alone to within the engine's tolerance: NLL 1e-9 relative, nu and phi
+1e-7, branch lengths 1e-6 relative (the shared forest's segment width and
+launch shapes change the order of sums over sites)."""
 import numpy as np
 import pytest
 import torch
 
 from laml_amd import sim
-from laml_amd.serving import PMMServer, SolveRequest, latency_summary, solve_alone
+from laml_amd.serving import PMMServer, SolveRequest, latency_summary, solve_alone, solve_batch
 
 pytestmark = pytest.mark.gpu
 
@@ -67,3 +71,34 @@ def test_concurrent_requests_with_chain_plans():
         assert r.nll == a[0] and r.nu == a[2] and r.phi == a[3] and r.iters == a[4]
         for x, y in zip(r.lengths, a[1]):
             np.testing.assert_array_equal(x, y)
+
+
+def _close(got, want):
+    nll, lengths, nu, phi, iters = got
+    assert abs(nll - want[0]) <= 1e-9 * abs(want[0])
+    assert abs(nu - want[2]) <= 1e-7 and abs(phi - want[3]) <= 1e-7
+    assert iters == want[4]
+    for x, y in zip(lengths, want[1]):
+        np.testing.assert_allclose(x, y, rtol=1e-6, atol=1e-10)
+
+
+def test_solve_batch_equals_solving_alone():
+    reqs = [r for r in _requests() if r.rid != 4]
+    alone = {r.rid: solve_alone(r) for r in reqs}
+    got = solve_batch(reqs)
+    for r, g in zip(reqs, got):
+        print("batched request", r.rid, g[0], alone[r.rid][0], g[4], alone[r.rid][4], flush=True)
+        _close(g, alone[r.rid])
+
+
+def test_server_batches_small_requests():
+    reqs = _requests()
+    alone = {r.rid: solve_alone(r) for r in reqs}
+    cap = 4 * sorted(r.bytes for r in reqs)[-2]   # all but the largest request are small
+    with PMMServer(slots=2, batch_bytes=cap, batch_max=8) as srv:
+        futs = [srv.submit(r) for r in reqs]
+        res = [f.result(timeout=300) for f in futs]
+    print("batches", srv.batches, flush=True)
+    assert max(srv.batches) > 1 and sum(srv.batches) == len(reqs)
+    for r in res:
+        _close((r.nll, r.lengths, r.nu, r.phi, r.iters), alone[r.rid])

@@ -1,6 +1,7 @@
 """Scheduling logic of the serving loop on the CPU (a stand-in runner that
 sleeps instead of solving): FIFO admission, the slot limit, the memory
-budget, overlap of solves, failures that do not stop the server."""
+budget, overlap of solves, failures that do not stop the server, and the
+batching of small requests."""
 import threading
 import time
 
@@ -13,12 +14,14 @@ class _Spec:
     pass
 
 
-def _req(rid, nbytes, work):
+def _req(rid, nbytes, work, max_iter=1000, tol=1e-8):
     r = SolveRequest.__new__(SolveRequest)
     r.spec = _Spec()
     r.spec.work = work
     r.rid = rid
     r.bytes = nbytes
+    r.max_iter = max_iter
+    r.tol = tol
     r.t_arrive = None
     return r
 
@@ -87,3 +90,88 @@ def test_late_arrivals_are_admitted_while_others_run():
         assert not f0.done()                  # it did not wait for request 0
         f0.result(timeout=10)
     assert started[1] < started[0] + 0.3
+
+
+def _batching_server(slots, budget, batch_bytes, batch_max, gate):
+    """A server whose single and batch runners wait on `gate` and record
+    the rids of each solve."""
+    solves = []
+    lock = threading.Lock()
+
+    def runner(req):
+        gate.wait(10)
+        with lock:
+            solves.append([req.rid])
+        return (float(req.rid), None, 0.0, 0.0, 1)
+
+    def batch_runner(reqs):
+        gate.wait(10)
+        with lock:
+            solves.append([r.rid for r in reqs])
+        return [(float(r.rid), None, 0.0, 0.0, 1) for r in reqs]
+
+    srv = PMMServer(slots=slots, hbm_budget=budget, runner=runner, batch_runner=batch_runner,
+                    batch_bytes=batch_bytes, batch_max=batch_max)
+    return srv, solves
+
+
+def test_small_requests_are_batched_in_queue_order():
+    gate = threading.Event()
+    srv, solves = _batching_server(slots=1, budget=1000, batch_bytes=100, batch_max=3, gate=gate)
+    with srv:
+        first = srv.submit(_req(0, 500, 0.0))      # holds the slot until the gate opens
+        time.sleep(0.05)
+        # small: bytes <= 25; 4 and 7 are large, 5 has another EM budget
+        sizes = {1: 10, 2: 20, 3: 25, 4: 30, 5: 10, 6: 5, 7: 200, 8: 5, 9: 20, 10: 20}
+        futs = {}
+        for rid in range(1, 11):
+            kw = {"max_iter": 50} if rid == 5 else {}
+            futs[rid] = srv.submit(_req(rid, sizes[rid], 0.0, **kw))
+        gate.set()
+        res = {rid: f.result(timeout=10) for rid, f in futs.items()}
+        first.result(timeout=10)
+    assert all(res[r].nll == float(r) for r in res)
+    # head 1 takes 2 and 3 (batch_max 3); 4 is large and runs alone; head 5
+    # (max_iter 50) finds no partner; head 6 takes 8 and 9 (7 is large,
+    # 10 would pass batch_max); then 7, then 10
+    assert solves == [[0], [1, 2, 3], [4], [5], [6, 8, 9], [7], [10]]
+    assert srv.batches == [1, 3, 1, 1, 3, 1, 1]
+    assert srv.in_use == 0 and srv.active == 0
+
+
+def test_batch_respects_byte_cap_and_budget():
+    gate = threading.Event()
+    srv, solves = _batching_server(slots=2, budget=60, batch_bytes=40, batch_max=16, gate=gate)
+    with srv:
+        futs = [srv.submit(_req(i, 10, 0.0)) for i in range(8)]
+        gate.set()
+        for f in futs:
+            f.result(timeout=10)
+    assert sorted(r for b in solves for r in b) == list(range(8))
+    assert all(len(b) * 10 <= 40 for b in solves)
+    assert srv.peak_in_use <= 60
+
+
+def test_failed_batch_fails_its_requests_only():
+    def batch_runner(reqs):
+        if any(r.rid == 1 for r in reqs):
+            raise RuntimeError("boom")
+        return [(1.0, None, 0.0, 0.0, 1) for _ in reqs]
+
+    gate = threading.Event()
+
+    def runner(req):
+        gate.wait(10)
+        return (1.0, None, 0.0, 0.0, 1)
+
+    with PMMServer(slots=1, hbm_budget=100, runner=runner, batch_runner=batch_runner, batch_bytes=40,
+                   batch_max=2) as srv:
+        f0 = srv.submit(_req(0, 50, 0.0))
+        time.sleep(0.05)
+        futs = [srv.submit(_req(i, 10, 0.0)) for i in range(1, 5)]
+        gate.set()
+        f0.result(timeout=10)
+        for f in futs[:2]:
+            with pytest.raises(RuntimeError):
+                f.result(timeout=10)
+        assert [f.result(timeout=10).nll for f in futs[2:]] == [1.0, 1.0]
