@@ -1115,173 +1115,313 @@ __global__ void __launch_bounds__(256) k_outside_rows_ch(DevForest F, int beg, i
 // passes.  The site's likelihood and prior values are read once per chain.
 // Partial counts go to the chunk's slot as in the level kernels with one
 // chunk per wavefront, so the results are the same bits.
-template <int K0, int K1>
-__device__ __forceinline__ void chain_site(const DevForest& F, const OutNode& n, int k, OutSite& x, double iL,
-                                           int hv, int raw0, int raw1, double q0, double q1, double4& au, int& Eo,
-                                           int& gu, double (&acc)[2][5], double& rt) {
-  const bool lf[2] = {K0 == kLeaf, K1 == kLeaf};
-  x.au = au;
-  x.Eo = Eo;
-  x.gu = gu;
-  const int raw[2] = {raw0, raw1};
+//
+// The steps are software-pipelined so that each waits for memory once, on
+// operations issued a whole step earlier.  The inside rows, branch
+// coefficients and prior gathers of step j + 1 and the codes of step j + 2
+// are copied to the wavefront's slice of LDS during step j by direct
+// global-to-LDS loads, which occupy no registers in flight.  At the top of
+// step j every outstanding access completes, step j's operands and step
+// j + 1's codes are read from LDS and the step records from the scalar
+// cache, and the next copies are issued; step j itself runs on registers.
+// No register written by a memory load is carried from one step to the
+// next, so the compiler never waits for one mid-step.  The vector memory
+// counter counts loads and stores together, so the partial counts of step j
+// are stored at the top of step j + 1, after its wait.
+
+// the codes (leaf) or az-partition labels (internal) of child c at row ro
+__device__ __forceinline__ const int16_t* chain_code_row(const DevForest& F, int c, int64_t ro) {
+  return (c < F.num_leaves ? F.codes : F.labels) + ro;
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+typedef const __attribute__((address_space(1))) void glb_void;
+
+// One wavefront's LDS slice: the next step's inside rows of both children,
+// the two 32-bit halves of both children's prior gathers, both children's
+// branch coefficients (10 words each), and two slots of codes (the aligned
+// 32-bit word holding each lane's 16-bit code, both children), used by
+// alternate steps.
+constexpr int kChainLdsBeta = 0;       // [2][64] double2
+constexpr int kChainLdsBE = 2048;      // [2][64] int32
+constexpr int kChainLdsQ = 2560;       // [2 children][lo, hi][64] uint32
+constexpr int kChainLdsCoef = 3584;    // [64] uint32 (20 used)
+constexpr int kChainLdsCode = 3840;    // [2 slots][2 children][64] uint32
+constexpr int kChainLdsBytes = 4864;
+
+// copy `size` (4 or 16) bytes at src (per lane) to dst + size * lane
+__device__ __forceinline__ void to_lds(const void* src, char* dst, unsigned size) {
+  if (size == 16)
+    __builtin_amdgcn_global_load_lds((glb_void*)src, (lds_void*)dst, 16, 0, 0);
+  else
+    __builtin_amdgcn_global_load_lds((glb_void*)src, (lds_void*)dst, 4, 0, 0);
+}
+template <typename T>
+__device__ __forceinline__ T lds_at(const char* base, unsigned byte_off) {
+  return *(const __attribute__((address_space(3))) T*)(base + byte_off);
+}
+template <>
+__device__ __forceinline__ double2 lds_at<double2>(const char* base, unsigned byte_off) {
+  const nt_double2 v = *(const __attribute__((address_space(3))) nt_double2*)(base + byte_off);
+  return make_double2(v.x, v.y);
+}
+
+// a wave-uniform double in scalar registers
+__device__ __forceinline__ double uniform_double(double x) {
+  const int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+  const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+  return __hiloint2double(hi, lo);
+}
+
+// every memory access issued so far has completed; nothing is moved across
+__device__ __forceinline__ void chain_wait_all() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_sched_barrier(0);
+}
+// every LDS read and scalar load issued so far has completed
+__device__ __forceinline__ void chain_wait_lgkm() {
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// the 16-bit code at site kk of a row, from the aligned word that holds it
+__device__ __forceinline__ int code_of_word(uint32_t wd, unsigned kk) {
+  return (int)(int16_t)(kk & 1u ? wd >> 16 : wd & 0xffffu);
+}
+
+// Copy step r's codes into code slot `slot`.
+__device__ __forceinline__ void chain_stage_codes(const DevForest& F, const ChainStep& r, unsigned kk, int slot,
+                                                  char* lds) {
+  const int16_t* row0 = chain_code_row(F, r.ch0, r.ro0) + (kk & ~1u);
+  const int16_t* row1 = chain_code_row(F, r.ch1, r.ro1) + (kk & ~1u);
+  to_lds(row0, lds + kChainLdsCode + 512 * slot, 4);
+  to_lds(row1, lds + kChainLdsCode + 512 * slot + 256, 4);
+}
+
+// Copy step r's operands: branch coefficients of both children, inside rows
+// of internal children, prior gathers of both from their raw codes (entry 0
+// for labels that name no edit state, so that every lane copies).
+__device__ __forceinline__ void chain_stage(const DevForest& F, const ChainStep& r, const int (&raw)[2], int qo,
+                                            unsigned kk, int lane, char* lds) {
+  {
+    const int c = lane < 10 ? r.ch0 : r.ch1;
+    const int wd = lane < 10 ? lane : min(lane - 10, 9);
+    to_lds(reinterpret_cast<const uint32_t*>(F.coef + c) + wd, lds + kChainLdsCoef, 4);
+  }
+  const int ch[2] = {r.ch0, r.ch1};
+  const int64_t ro[2] = {r.ro0, r.ro1};
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
-    if (lf[i]) {
-      x.code[i] = raw[i];
-      leaf_beta(raw[i], n.phi, x.lab[i], x.b0[i], x.bA[i]);
-      x.E[i] = 0;
-    } else {
-      x.code[i] = 0;
-      x.lab[i] = raw[i];
-      const double2 bb = row_at(n.c[i].beta, k);
-      x.b0[i] = bb.x;
-      x.bA[i] = bb.y;
-      x.E[i] = row_at(n.c[i].bE, k);
+    const bool leaf = ch[i] < F.num_leaves;
+    if (!leaf) {
+      to_lds(&row_at(F.beta + ro[i], kk), lds + kChainLdsBeta + 1024 * i, 16);
+      to_lds(&row_at(F.bE + ro[i], kk), lds + kChainLdsBE + 256 * i, 4);
     }
+    const int lab = leaf ? code_label(raw[i]) : raw[i];
+    const uint32_t* g = reinterpret_cast<const uint32_t*>(F.prior + (lab > 0 ? qo + lab : 0));
+    to_lds(g, lds + kChainLdsQ + 512 * i, 4);
+    to_lds(g + 1, lds + kChainLdsQ + 512 * i + 256, 4);
   }
-  x.q[0] = q0;
-  x.q[1] = q1;
+}
+
+// One chain step at one site, on registers: the children's raw row values
+// (leaf codes or labels), gathered priors (entry for labels that name no
+// edit state: ignored), inside rows (ignored for leaves) and branch
+// coefficients.  The heavy child's outside state replaces (au, Eo, gu); the
+// light internal child's is stored at row ro[light].
+template <int K0, int K1, typename Stage>
+__device__ __forceinline__ void chain_compute(const DevForest& F, const char* lds, int lane, const Coef (&cf)[2],
+                                              double phi, int hv, double iL, int LE, double qs, const int (&raw)[2],
+                                              const int64_t (&ro)[2], unsigned kk, bool on, bool stream,
+                                              Stage&& stage, double4& au, int& Eo, int& gu, double (&acc)[2][5],
+                                              double& rt) {
+  const bool lf[2] = {K0 == kLeaf, K1 == kLeaf};
+  int lab[2], E[2], code[2];
+  double b0[2], bA[2], q[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const uint32_t glo = lds_at<uint32_t>(lds, kChainLdsQ + 512 * i + 4 * lane);
+    const uint32_t ghi = lds_at<uint32_t>(lds, kChainLdsQ + 512 * i + 256 + 4 * lane);
+    const double g = __hiloint2double((int)ghi, (int)glo);
+    if (lf[i]) {
+      code[i] = raw[i];
+      leaf_beta(raw[i], phi, lab[i], b0[i], bA[i]);
+      E[i] = 0;
+    } else {
+      code[i] = 0;
+      lab[i] = raw[i];
+      const double2 bb = lds_at<double2>(lds, kChainLdsBeta + 1024 * i + 16 * lane);
+      b0[i] = bb.x;
+      bA[i] = bb.y;
+      E[i] = lds_at<int>(lds, kChainLdsBE + 256 * i + 4 * lane);
+    }
+    q[i] = lab[i] > 0 ? g : qs;
+  }
+  // the slice is read: the next step's copies may overwrite it
+  chain_wait_lgkm();
+  stage();
+  if (!on) return;
   Eps ep[2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) ep[i] = edge(x.lab[i], x.b0[i], x.bA[i], x.E[i], x.q[i], n.cf[i]);
-  post_site(x.au, x.Eo, x.gu, iL, x.LE, x.qs, n.phi, x.lab, x.b0, x.bA, x.E, x.code, x.q, ep, n.cf, lf, acc, rt,
+  for (int i = 0; i < 2; ++i) ep[i] = edge(lab[i], b0[i], bA[i], E[i], q[i], cf[i]);
+  const double4 au0 = au;
+  const int Eo0 = Eo, gu0 = gu;
+  post_site(au0, Eo0, gu0, iL, LE, qs, phi, lab, b0, bA, E, code, q, ep, cf, lf, acc, rt,
             [&](int i, const double4& a, int ec, int gg) {
               if (i == hv) {
                 au = a;
                 Eo = ec;
                 gu = gg;
               } else {
-                if (n.stream)
-                  store_stream(&row_at(n.ca[i], k), a);
+                if (stream)
+                  store_stream(&row_at(F.alpha + ro[i], kk), a);
                 else
-                  row_at(n.ca[i], k) = a;
-                row_at(n.caE[i], k) = ec;
-                row_at(n.caG[i], k) = (int16_t)gg;
+                  row_at(F.alpha + ro[i], kk) = a;
+                row_at(F.aE + ro[i], kk) = ec;
+                row_at(F.aG + ro[i], kk) = (int16_t)gg;
               }
             });
-}
-
-// the prior of the state a child's raw row value (leaf code or az-partition
-// label) names at a site
-__device__ __forceinline__ double chain_q(const DevForest& F, bool leaf, int raw, int qo, double qs) {
-  const int lab = leaf ? code_label(raw) : raw;
-  return lab > 0 ? row_at(F.prior, (unsigned)(qo + lab)) : qs;
-}
-
-// The per-node values of one chain step, from its record: children, their
-// kinds and rows (no level checkpointing: inside rows sit at row_off), the
-// branch coefficients; tree-wide values (phi, site pointers) stay as set.
-__device__ __forceinline__ void chain_step_node(const DevForest& F, const ChainStep& r, OutNode& n) {
-  const int ch[2] = {r.ch0, r.ch1};
-  const int64_t ro[2] = {r.ro0, r.ro1};
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    n.ch[i] = ch[i];
-    n.kind[i] = ch[i] < F.num_leaves ? kLeaf : kInternal;
-    n.cf[i] = uniform_load(F.coef + ch[i]);
-    n.c[i].code = (n.kind[i] == kLeaf ? F.codes : F.labels) + ro[i];
-    n.c[i].beta = F.beta + ro[i];
-    n.c[i].bE = F.bE + ro[i];
-    n.ca[i] = F.alpha + ro[i];
-    n.caE[i] = F.aE + ro[i];
-    n.caG[i] = F.aG + ro[i];
-  }
 }
 
 // Chains [cbeg, cbeg + cnt) of a chain plan: chain c's steps are
 // steps[chains[c].x .. chains[c].x + chains[c].y), head first.  Wavefront w
 // runs chain w / max_chunks at chunk w % max_chunks, so that every chunk of
-// the (longest, first) chains starts first.  Each step's record is loaded
-// one step ahead, so a step's row loads need no dependent metadata loads.
-// Grid: ceil(cnt * max_chunks / 4) blocks of 256; the head's outside row
-// must be complete.
+// the (longest, first) chains starts first.  Lanes past the tree's sites
+// read its last site (every address stays inside the rows) and neither
+// store nor add to the counts.  Grid: ceil(cnt * max_chunks / 4) blocks of
+// 256; the head's outside row must be complete.
 __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* __restrict__ chains, int cbeg,
                                                         int cnt, const ChainStep* __restrict__ steps,
                                                         const int32_t* __restrict__ done) {
+  __shared__ __align__(16) char lds_all[4 * kChainLdsBytes];
   const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   const int wv = w / F.max_chunks;
   if (wv >= cnt) return;
-  const int2 cd = chains[cbeg + wv];
-  const int nb = __builtin_amdgcn_readfirstlane(cd.x);
-  const int len = __builtin_amdgcn_readfirstlane(cd.y);
-  ChainStep r = steps[nb];
-  const int t = F.tree_of[r.u];
+  const int2 cd = uniform_load(chains + cbeg + wv);
+  const int nb = cd.x, len = cd.y;
+  const ChainStep* cs = steps + nb;
+  ChainStep r0 = uniform_load(cs);
+  const int t = F.tree_of[r0.u];
   const int p = F.tree_prob[t];
   const int K = F.tree_nsites[t];
   const int chunk = w - wv * F.max_chunks;
   if (chunk * 64 >= K) return;
   if (done && done[p]) return;
+  char* lds = lds_all + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * kChainLdsBytes;
   const int lane = threadIdx.x & 63;
   const int k = chunk * 64 + lane;
   const bool on = k < K;
-  OutNode n;
-  out_node_load<false>(F, r.u, t, p, K, n);
-  OutSite x;
-  double iL = 0.0;
-  double4 au = make_double4(0.0, 0.0, 0.0, 0.0);
-  int Eo = 0, gu = 0;
-  if (on) {
-    au = row_at(n.a, k);
-    Eo = row_at(n.aE, k);
-    gu = row_at(n.aG, k);
-    x.iL = row_at(n.iLm, k);
-    x.LE = row_at(n.LE, k);
-    x.qo = row_at(n.qoff, k) - 1;
-    x.qs = row_at(n.qsum, k);
-    iL = x.iL;
+  const unsigned kk = on ? k : K - 1;
+  const int sb = F.tree_site_base[t];
+  const double phi = F.phi[p];
+  const bool stream = F.stream_rows != 0;
+  const double iL = row_at(F.iLm + sb, kk);
+  const int LE = row_at(F.LE + sb, kk);
+  const int qo = row_at(F.site_qoff + sb, kk) - 1;
+  const double qs = row_at(F.site_qsum + sb, kk);
+  const int64_t ru = F.row_off[r0.u];
+  double4 au = row_at(F.alpha + ru, kk);
+  int Eo = row_at(F.aE + ru, kk);
+  int gu = row_at(F.aG + ru, kk);
+
+  // prologue: step 0's operands and codes and step 1's codes staged
+  {
+    int raw0[2];
+    raw0[0] = row_at(chain_code_row(F, r0.ch0, r0.ro0), kk);
+    raw0[1] = row_at(chain_code_row(F, r0.ch1, r0.ro1), kk);
+    chain_stage_codes(F, r0, kk, 0, lds);
+    if (len > 1) chain_stage_codes(F, uniform_load(cs + 1), kk, 1, lds);
+    chain_stage(F, r0, raw0, qo, kk, lane, lds);
   }
-  // The children's codes / labels of the next step are loaded at the start
-  // of a step (its record is loaded two steps ahead) and their priors (the
-  // gathers that need those values) at its end, so a step's own loads are
-  // the inside rows only.
-  int raw0 = 0, raw1 = 0;
-  double q0 = 0.0, q1 = 0.0;
-  if (on) {
-    raw0 = row_at(n.c[0].code, k);
-    raw1 = row_at(n.c[1].code, k);
-    q0 = chain_q(F, n.kind[0] == kLeaf, raw0, x.qo, x.qs);
-    q1 = chain_q(F, n.kind[1] == kLeaf, raw1, x.qo, x.qs);
-  }
-  ChainStep rn = r;
-  if (len > 1) rn = steps[nb + 1];
+  int touch = 0;  // first word of the record four steps ahead (brings its line into the scalar cache)
+  if (len > 2) touch = uniform_load(&cs[min(len - 1, 4)].u);
+
+  double* pp = nullptr;  // partial count held back from the previous step
+  double pv = 0.0;
   for (int j = 0;;) {
+    chain_wait_all();
+    asm volatile("" ::"s"(touch));
+    if (pp) *pp = pv;
     const bool more = j + 1 < len;
-    ChainStep rnn = rn;
-    if (j + 2 < len) rnn = uniform_load(steps + nb + j + 2);
-    int rawn0 = 0, rawn1 = 0;
-    if (more && on) {
-      rawn0 = row_at((rn.ch0 < F.num_leaves ? F.codes : F.labels) + rn.ro0, k);
-      rawn1 = row_at((rn.ch1 < F.num_leaves ? F.codes : F.labels) + rn.ro1, k);
+    const bool more2 = j + 2 < len;
+    r0 = uniform_load(cs + j);
+    const ChainStep r1 = uniform_load(cs + (more ? j + 1 : j));
+    const ChainStep r2 = uniform_load(cs + (more2 ? j + 2 : j));
+    if (j + 4 < len) touch = uniform_load(&cs[j + 4].u);
+    // this step's operands and the next step's codes, from the slice
+    // (wave-uniform: moved to scalar registers)
+    Coef cf[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      cf[i].p1n = uniform_double(lds_at<double>(lds, kChainLdsCoef + 40 * i));
+      cf[i].mut = uniform_double(lds_at<double>(lds, kChainLdsCoef + 40 * i + 8));
+      cf[i].pn = uniform_double(lds_at<double>(lds, kChainLdsCoef + 40 * i + 16));
+      cf[i].msil = uniform_double(lds_at<double>(lds, kChainLdsCoef + 40 * i + 24));
+      cf[i].esil = __builtin_amdgcn_readfirstlane(lds_at<int>(lds, kChainLdsCoef + 40 * i + 32));
+      cf[i].pad = 0;
     }
+    const int slot = j & 1;
+    const int raw[2] = {code_of_word(lds_at<uint32_t>(lds, kChainLdsCode + 512 * slot + 4 * lane), kk),
+                        code_of_word(lds_at<uint32_t>(lds, kChainLdsCode + 512 * slot + 256 + 4 * lane), kk)};
+    const int rawn[2] = {code_of_word(lds_at<uint32_t>(lds, kChainLdsCode + 512 * (slot ^ 1) + 4 * lane), kk),
+                         code_of_word(lds_at<uint32_t>(lds, kChainLdsCode + 512 * (slot ^ 1) + 256 + 4 * lane), kk)};
+    // operands of step j + 1 and codes of step j + 2 (into this step's
+    // slot), issued once this step's operands are read from the slice
+    auto stage = [&]() {
+      if (more) {
+        chain_stage(F, r1, rawn, qo, kk, lane, lds);
+        if (more2) chain_stage_codes(F, r2, kk, slot, lds);
+      }
+    };
+    // step j
     double acc[2][5];
     zero_acc(acc);
     double rt = 0.0;
-    if (on) {
-      if (n.kind[0] == kLeaf) {
-        if (n.kind[1] == kLeaf)
-          chain_site<kLeaf, kLeaf>(F, n, k, x, iL, r.hv, raw0, raw1, q0, q1, au, Eo, gu, acc, rt);
-        else
-          chain_site<kLeaf, kInternal>(F, n, k, x, iL, r.hv, raw0, raw1, q0, q1, au, Eo, gu, acc, rt);
-      } else {
-        if (n.kind[1] == kLeaf)
-          chain_site<kInternal, kLeaf>(F, n, k, x, iL, r.hv, raw0, raw1, q0, q1, au, Eo, gu, acc, rt);
-        else
-          chain_site<kInternal, kInternal>(F, n, k, x, iL, r.hv, raw0, raw1, q0, q1, au, Eo, gu, acc, rt);
+    const int hv = r0.hv;
+    const int64_t ro[2] = {r0.ro0, r0.ro1};
+    if (r0.ch0 < F.num_leaves) {
+      if (r0.ch1 < F.num_leaves)
+        chain_compute<kLeaf, kLeaf>(F, lds, lane, cf, phi, hv, iL, LE, qs, raw, ro, kk, on, stream, stage, au, Eo,
+                                    gu, acc, rt);
+      else
+        chain_compute<kLeaf, kInternal>(F, lds, lane, cf, phi, hv, iL, LE, qs, raw, ro, kk, on, stream, stage, au,
+                                        Eo, gu, acc, rt);
+    } else {
+      if (r0.ch1 < F.num_leaves)
+        chain_compute<kInternal, kLeaf>(F, lds, lane, cf, phi, hv, iL, LE, qs, raw, ro, kk, on, stream, stage, au,
+                                        Eo, gu, acc, rt);
+      else
+        chain_compute<kInternal, kInternal>(F, lds, lane, cf, phi, hv, iL, LE, qs, raw, ro, kk, on, stream, stage,
+                                            au, Eo, gu, acc, rt);
+    }
+    // partial counts: one butterfly sum of the eleven values; lane 16r + i
+    // (i < 3) holds total 3r + i (counts 0..4 of child 0, of child 1, the
+    // dropout sum), stored at the top of the next step
+    double v[12];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      v[m] = acc[0][m];
+      v[5 + m] = acc[1][m];
+    }
+    v[10] = rt;
+    v[11] = 0.0;
+    wave_sum12(v);
+    {
+      const int rr = lane >> 4, i = lane & 15;
+      const int vi = 3 * rr + i;
+      pv = i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
+      pp = nullptr;
+      if (i < 3 && vi < 10) {
+        const int c = vi >= 5 ? 1 : 0;
+        pp = F.cpart + ((int64_t)(c ? r0.ch1 : r0.ch0) * F.max_chunks + chunk) * 5 + (vi - 5 * c);
+      } else if (i < 3 && vi == 10) {
+        pp = F.rt_part + (int64_t)r0.u * F.max_chunks + chunk;
       }
     }
-    out_write_partials<64>(F, r.u, n.ch, lane, chunk, chunk + 1, acc, rt);
     if (!more) break;
     ++j;
-    r = rn;
-    rn = rnn;
-    chain_step_node(F, r, n);
-    if (on) {
-      raw0 = rawn0;
-      raw1 = rawn1;
-      q0 = chain_q(F, n.kind[0] == kLeaf, raw0, x.qo, x.qs);
-      q1 = chain_q(F, n.kind[1] == kLeaf, raw1, x.qo, x.qs);
-    }
   }
+  if (pp) *pp = pv;
 }
 
 template <int W>
