@@ -731,7 +731,8 @@ def main():
     ap.add_argument("--save-plan", default=None,
                     help="write the batch forest's autotuned launch plan to this JSON file")
     ap.add_argument("--plan", default=None,
-                    help="apply the launch plan in this JSON file (written by --save-plan) instead of autotuning")
+                    help="apply the launch plan in this JSON file (written by --save-plan) instead of autotuning; "
+                         "'profile': the plan the committed profile summary of the workload was measured with")
     ap.add_argument("--no-extra-legs", action="store_true",
                     help="skip the serving-latency and batched-scoring legs of the default run")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -778,7 +779,7 @@ def main():
         phi = torch.full((len(problems),), 0.5, dtype=torch.float64, device=dev)
         if tune and args.plan:
             from laml_amd.autotune import plan_from_dict
-            plan = plan_from_dict(json.load(open(args.plan)), F)
+            plan = plan_from_dict(profile_plan() if args.plan == "profile" else json.load(open(args.plan)), F)
             plan.apply(F)
             plan_desc.update(plan.describe())
             log("launch plan from %s: %s" % (args.plan, plan.describe().get("mode")))

@@ -544,7 +544,13 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
     if (d->site_qsum) std::copy(d->site_qsum, d->site_qsum + S, qs.begin());
     UP(site_qsum, qs.data(), qs.size());
   }
-  UP(prior, d->prior, d->site_qoff[S]);
+  {
+    // two entries of padding: the outside chain kernel gathers 16 bytes at a
+    // prior entry (entry 0 for labels that name no edit state)
+    std::vector<double> pr((size_t)d->site_qoff[S] + 2, 0.0);
+    std::copy(d->prior, d->prior + d->site_qoff[S], pr.begin());
+    UP(prior, pr.data(), pr.size());
+  }
   UP(codes, codes_p.data(), leaf_rows);
 #undef UP
   {

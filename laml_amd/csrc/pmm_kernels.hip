@@ -1138,23 +1138,33 @@ typedef __attribute__((address_space(3))) void lds_void;
 typedef const __attribute__((address_space(1))) void glb_void;
 
 // One wavefront's LDS slice: the next step's inside rows of both children,
-// the two 32-bit halves of both children's prior gathers, both children's
-// branch coefficients (10 words each), and two slots of codes (the aligned
-// 32-bit word holding each lane's 16-bit code, both children), used by
-// alternate steps.
+// both children's prior gathers (16 bytes from the gathered entry, the
+// first 8 used: one access per lane and child), both children's branch
+// coefficients (10 words each), and two slots of codes (the aligned 32-bit
+// word holding each lane's 16-bit code, both children), used by alternate
+// steps.
 constexpr int kChainLdsBeta = 0;       // [2][64] double2
 constexpr int kChainLdsBE = 2048;      // [2][64] int32
-constexpr int kChainLdsQ = 2560;       // [2 children][lo, hi][64] uint32
-constexpr int kChainLdsCoef = 3584;    // [64] uint32 (20 used)
-constexpr int kChainLdsCode = 3840;    // [2 slots][2 children][64] uint32
-constexpr int kChainLdsBytes = 4864;
+constexpr int kChainLdsQ = 2560;       // [2][64] 16 bytes
+constexpr int kChainLdsCoef = 4608;    // [64] uint32 (20 used)
+constexpr int kChainLdsCode = 4864;    // [2 slots][2 children][64] uint32
+constexpr int kChainLdsBytes = 5888;
 
-// copy `size` (4 or 16) bytes at src (per lane) to dst + size * lane
-__device__ __forceinline__ void to_lds(const void* src, char* dst, unsigned size) {
-  if (size == 16)
-    __builtin_amdgcn_global_load_lds((glb_void*)src, (lds_void*)dst, 16, 0, 0);
-  else
-    __builtin_amdgcn_global_load_lds((glb_void*)src, (lds_void*)dst, 4, 0, 0);
+// copy `size` (4 or 16) bytes at src (per lane) to dst + size * lane; rows
+// read once per pass carry the non-temporal hint, so that the per-site
+// tables and priors every chain of a tree reads stay in the L2 cache
+__device__ __forceinline__ void to_lds(const void* src, char* dst, unsigned size, bool stream = false) {
+  if (size == 16) {
+    if (stream)
+      __builtin_amdgcn_global_load_lds((glb_void*)src, (lds_void*)dst, 16, 0, 2);
+    else
+      __builtin_amdgcn_global_load_lds((glb_void*)src, (lds_void*)dst, 16, 0, 0);
+  } else {
+    if (stream)
+      __builtin_amdgcn_global_load_lds((glb_void*)src, (lds_void*)dst, 4, 0, 2);
+    else
+      __builtin_amdgcn_global_load_lds((glb_void*)src, (lds_void*)dst, 4, 0, 0);
+  }
 }
 template <typename T>
 __device__ __forceinline__ T lds_at(const char* base, unsigned byte_off) {
@@ -1194,8 +1204,8 @@ __device__ __forceinline__ void chain_stage_codes(const DevForest& F, const Chai
                                                   char* lds) {
   const int16_t* row0 = chain_code_row(F, r.ch0, r.ro0) + (kk & ~1u);
   const int16_t* row1 = chain_code_row(F, r.ch1, r.ro1) + (kk & ~1u);
-  to_lds(row0, lds + kChainLdsCode + 512 * slot, 4);
-  to_lds(row1, lds + kChainLdsCode + 512 * slot + 256, 4);
+  to_lds(row0, lds + kChainLdsCode + 512 * slot, 4, true);
+  to_lds(row1, lds + kChainLdsCode + 512 * slot + 256, 4, true);
 }
 
 // Copy step r's operands: branch coefficients of both children, inside rows
@@ -1214,13 +1224,11 @@ __device__ __forceinline__ void chain_stage(const DevForest& F, const ChainStep&
   for (int i = 0; i < 2; ++i) {
     const bool leaf = ch[i] < F.num_leaves;
     if (!leaf) {
-      to_lds(&row_at(F.beta + ro[i], kk), lds + kChainLdsBeta + 1024 * i, 16);
-      to_lds(&row_at(F.bE + ro[i], kk), lds + kChainLdsBE + 256 * i, 4);
+      to_lds(&row_at(F.beta + ro[i], kk), lds + kChainLdsBeta + 1024 * i, 16, true);
+      to_lds(&row_at(F.bE + ro[i], kk), lds + kChainLdsBE + 256 * i, 4, true);
     }
     const int lab = leaf ? code_label(raw[i]) : raw[i];
-    const uint32_t* g = reinterpret_cast<const uint32_t*>(F.prior + (lab > 0 ? qo + lab : 0));
-    to_lds(g, lds + kChainLdsQ + 512 * i, 4);
-    to_lds(g + 1, lds + kChainLdsQ + 512 * i + 256, 4);
+    to_lds(F.prior + (lab > 0 ? qo + lab : 0), lds + kChainLdsQ + 1024 * i, 16);
   }
 }
 
@@ -1240,9 +1248,7 @@ __device__ __forceinline__ void chain_compute(const DevForest& F, const char* ld
   double b0[2], bA[2], q[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
-    const uint32_t glo = lds_at<uint32_t>(lds, kChainLdsQ + 512 * i + 4 * lane);
-    const uint32_t ghi = lds_at<uint32_t>(lds, kChainLdsQ + 512 * i + 256 + 4 * lane);
-    const double g = __hiloint2double((int)ghi, (int)glo);
+    const double g = lds_at<double>(lds, kChainLdsQ + 1024 * i + 16 * lane);
     if (lf[i]) {
       code[i] = raw[i];
       leaf_beta(raw[i], phi, lab[i], b0[i], bA[i]);
