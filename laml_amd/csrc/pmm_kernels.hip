@@ -392,6 +392,69 @@ __device__ __forceinline__ void inside_row_run(const DevForest& F, const InNode&
   }
 }
 
+// Both children leaves (the cherry level, whose rows are mostly written):
+// the codes, labels and site values of the wavefront's chunks (at most
+// kLeafGroup, their number a template argument so that the code is
+// straight-line) are loaded first, then their prior gathers, then the
+// arithmetic and stores, so that a wavefront waits twice instead of twice
+// per chunk (a load issued after a store waits for the store too).  The
+// per-chunk state is kept compact (raw codes until the arithmetic); same
+// operations as inside_site, so the same bits.
+constexpr int kLeafGroup = 8;
+template <int NG>
+__device__ __forceinline__ void inside_row_run_leaves_n(const DevForest& F, const InNode& n, int c0, int lane) {
+  // lanes past the tree's sites load its last site, so that no load depends
+  // on the lane mask
+  int lv[NG], ca[NG], cb[NG], qo[NG];
+  double qs[NG], ga[NG], gb[NG];
+#pragma unroll
+  for (int j = 0; j < NG; ++j) {
+    const unsigned kk = min((c0 + j) * 64 + lane, n.K - 1);
+    lv[j] = row_at(n.lab, kk);
+    qo[j] = row_at(n.qoff, kk);
+    qs[j] = row_at(n.qsum, kk);
+    ca[j] = row_at(n.c[0].code, kk);
+    cb[j] = row_at(n.c[1].code, kk);
+  }
+#pragma unroll
+  for (int j = 0; j < NG; ++j) {
+    const int la = code_label(ca[j]), lb = code_label(cb[j]);
+    ga[j] = row_at(F.prior, (unsigned)(la > 0 ? qo[j] - 1 + la : 0));
+    gb[j] = row_at(F.prior, (unsigned)(lb > 0 ? qo[j] - 1 + lb : 0));
+  }
+#pragma unroll
+  for (int j = 0; j < NG; ++j) {
+    const int k = (c0 + j) * 64 + lane;
+    if (k < n.K) {
+      InSite x;
+      x.lv = lv[j];
+      x.qo = qo[j] - 1;
+      x.qs = qs[j];
+      x.ca = ca[j];
+      x.cb = cb[j];
+      leaf_beta(ca[j], n.phi, x.la, x.a0, x.aA);
+      leaf_beta(cb[j], n.phi, x.lb, x.b0, x.bA);
+      x.Ea = 0;
+      x.Eb = 0;
+      x.qa = x.la > 0 ? ga[j] : x.qs;
+      x.qb = x.lb > 0 ? gb[j] : x.qs;
+      inside_finish(n, k, x);
+    }
+  }
+}
+__device__ __forceinline__ void inside_row_run_leaves(const DevForest& F, const InNode& n, int c0, int c1, int lane) {
+  switch (c1 - c0) {
+    case 1: inside_row_run_leaves_n<1>(F, n, c0, lane); break;
+    case 2: inside_row_run_leaves_n<2>(F, n, c0, lane); break;
+    case 3: inside_row_run_leaves_n<3>(F, n, c0, lane); break;
+    case 4: inside_row_run_leaves_n<4>(F, n, c0, lane); break;
+    case 5: inside_row_run_leaves_n<5>(F, n, c0, lane); break;
+    case 6: inside_row_run_leaves_n<6>(F, n, c0, lane); break;
+    case 7: inside_row_run_leaves_n<7>(F, n, c0, lane); break;
+    default: inside_row_run_leaves_n<8>(F, n, c0, lane); break;
+  }
+}
+
 // The wave's node: entry beg + wv of a node list, or of the level range.
 // Returns -1 past the end.
 __device__ __forceinline__ int row_node(int beg, int cnt, const int32_t* __restrict__ nodes) {
@@ -422,6 +485,30 @@ __global__ void __launch_bounds__(256) k_inside_rows(DevForest F, int beg, int c
     if (n.kind[1] == kLeaf) inside_row_run<kInternal, kLeaf>(F, n, c0, c1, lane);
     else inside_row_run<kInternal, kInternal>(F, n, c0, c1, lane);
   }
+}
+
+// The same for level 1, whose nodes have two leaf children (a separate
+// kernel: its loads-first path needs more registers than the other levels'
+// kernel should carry).
+__global__ void __launch_bounds__(256) k_inside_rows_leaves(DevForest F, int beg, int cnt, int cpw,
+                                                             const int32_t* __restrict__ done) {
+  const int v = row_node(beg, cnt, nullptr);
+  if (v < 0) return;
+  const int lane = threadIdx.x & 63;
+  const NodeRec r = F.nrec[v - F.num_leaves];
+  const int nch = (r.K + 63) >> 6;
+  const int c0 = (int)blockIdx.y * cpw;
+  if (c0 >= nch) return;
+  if (done && done[r.p]) return;
+  const int c1 = min(c0 + cpw, nch);
+  InNode n;
+  in_node_rec(F, r, n);
+  if (n.kind[0] == kLeaf && n.kind[1] == kLeaf) {
+    if (c1 - c0 <= kLeafGroup) inside_row_run_leaves(F, n, c0, c1, lane);
+    else inside_row_run<kLeaf, kLeaf>(F, n, c0, c1, lane);
+  } else if (n.kind[0] == kLeaf) inside_row_run<kLeaf, kInternal>(F, n, c0, c1, lane);
+  else if (n.kind[1] == kLeaf) inside_row_run<kInternal, kLeaf>(F, n, c0, c1, lane);
+  else inside_row_run<kInternal, kInternal>(F, n, c0, c1, lane);
 }
 
 // Nodes with at least one fused cherry child (a separate kernel, so that the
@@ -2623,7 +2710,11 @@ static void inside_level(const DevForest& F, const std::vector<int32_t>& lvl, in
   if (cnt <= 0) return;
   if (W == 64 && !cpw.empty() && cpw[h] > 0) {
     const int c = cpw[h];
-    launch_inside_rows(dim3((cnt + 3) / 4, (F.max_chunks + c - 1) / c), s, F, beg, cnt, c, done, nullptr);
+    const dim3 grid((cnt + 3) / 4, (F.max_chunks + c - 1) / c);
+    if (h == 1)
+      hipLaunchKernelGGL(k_inside_rows_leaves, grid, dim3(256), 0, s, F, beg, cnt, c, done);
+    else
+      launch_inside_rows(grid, s, F, beg, cnt, c, done, nullptr);
     return;
   }
   LAML_DISPATCH_W(W, {
