@@ -17,8 +17,11 @@ else is being served; the tests check that bit for bit.
 
 With batching on (``batch_bytes`` > 0), a slot whose next request is small
 (footprint at most ``batch_bytes`` / 4) also takes the other small queued
-requests with the same EM budget, in queue order, up to ``batch_bytes`` and
-``batch_max`` requests, and solves them as one device-resident forest: one
+requests with the same EM budget and the same row width (sites padded to 8,
+16, 32 or 64; a shared forest pads every row to its widest, so at one width
+its footprint is the sum of the requests'), in queue order, up to
+``batch_bytes`` and ``batch_max`` requests, and solves them as one
+device-resident forest: one
 problem per request, one launch sequence for all of them, each problem
 stopping at its own convergence test.  Small requests then share launches
 instead of each paying a full launch sequence; each result equals the
@@ -76,6 +79,9 @@ class SolveRequest:
         self.tol = float(tol)
         self.rid = next(SolveRequest._ids) if rid is None else rid
         self.bytes = estimate_forest_bytes([spec]) if nbytes is None else int(nbytes)
+        # rows are padded to this many sites; requests share a forest only
+        # at the same width, so that its footprint is the sum of theirs
+        self.width = segment_width(spec.sites.K)
         self.t_arrive = None
 
 
@@ -201,7 +207,8 @@ class PMMServer:
     def _admit(self):
         # FIFO: the head request runs as soon as a slot and its memory are
         # free; a small head takes the other small queued requests with its
-        # EM budget along (in queue order), within the batch limits
+        # EM budget and row width along (in queue order), within the batch
+        # limits
         while True:
             if self._closing and not self._queue:
                 return None
@@ -211,10 +218,12 @@ class PMMServer:
                 total = req.bytes
                 if self._small(req):
                     keep = collections.deque()
+                    width = getattr(req, "width", None)
                     while self._queue:
                         r2, f2 = self._queue.popleft()
                         if (len(items) < self.batch_max and self._small(r2) and r2.max_iter == req.max_iter
-                                and r2.tol == req.tol and total + r2.bytes <= self.batch_bytes
+                                and r2.tol == req.tol and getattr(r2, "width", None) == width
+                                and total + r2.bytes <= self.batch_bytes
                                 and self.in_use + total + r2.bytes <= self.budget):
                             items.append((r2, f2))
                             total += r2.bytes

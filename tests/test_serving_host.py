@@ -14,7 +14,7 @@ class _Spec:
     pass
 
 
-def _req(rid, nbytes, work, max_iter=1000, tol=1e-8):
+def _req(rid, nbytes, work, max_iter=1000, tol=1e-8, width=64):
     r = SolveRequest.__new__(SolveRequest)
     r.spec = _Spec()
     r.spec.work = work
@@ -22,6 +22,7 @@ def _req(rid, nbytes, work, max_iter=1000, tol=1e-8):
     r.bytes = nbytes
     r.max_iter = max_iter
     r.tol = tol
+    r.width = width
     r.t_arrive = None
     return r
 
@@ -175,3 +176,20 @@ def test_failed_batch_fails_its_requests_only():
             with pytest.raises(RuntimeError):
                 f.result(timeout=10)
         assert [f.result(timeout=10).nll for f in futs[2:]] == [1.0, 1.0]
+
+
+def test_batches_keep_one_row_width():
+    """Requests whose rows pad to different widths are not solved together
+    (a shared forest would pad every row to the widest)."""
+    gate = threading.Event()
+    srv, solves = _batching_server(slots=1, budget=1000, batch_bytes=100, batch_max=8, gate=gate)
+    with srv:
+        first = srv.submit(_req(0, 500, 0.0))
+        time.sleep(0.05)
+        widths = {1: 64, 2: 32, 3: 64, 4: 32, 5: 8}
+        futs = [srv.submit(_req(r, 10, 0.0, width=w)) for r, w in widths.items()]
+        gate.set()
+        for f in futs:
+            f.result(timeout=10)
+        first.result(timeout=10)
+    assert solves == [[0], [1, 3], [2, 4], [5]]
