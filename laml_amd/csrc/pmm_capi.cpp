@@ -290,26 +290,29 @@ int validate(const laml_forest_desc* d, Derived& D) {
   if (d->site_qoff[0] != 0) return fail(LAML_EINVAL, "site_qoff[0] != 0");
   // kernels index the prior table with 32-bit byte offsets
   if (d->site_qoff[S] >= (1 << 28)) return fail(LAML_EINVAL, "prior table too large (>= 2^28 entries)");
-  for (int s = 0; s < S; ++s)
-    if (d->site_qoff[s + 1] < d->site_qoff[s]) return fail(LAML_EINVAL, "site_qoff not monotone");
-  for (int v = 0; v < n; ++v) {
-    const int t = d->tree_of[v];
-    const int K = d->tree_nsites[t];
-    for (int k = 0; k < K; ++k) {
-      const int c = d->leaf_codes[row_off[v] + k];
-      const int s = tree_site_base[t] + k;
-      const int m = d->site_qoff[s + 1] - d->site_qoff[s];
-      if (c < -2 || c > m) return fail(LAML_EINVAL, "leaf code out of range for its site");
-    }
+  // alphabet size of every site
+  std::vector<int32_t> alpha_size(std::max(S, 1));
+  for (int s = 0; s < S; ++s) {
+    alpha_size[s] = d->site_qoff[s + 1] - d->site_qoff[s];
+    if (alpha_size[s] < 0) return fail(LAML_EINVAL, "site_qoff not monotone");
   }
-  // entries observed as 0 or an edit, per problem (phi gradient)
+  // leaf codes in range for their sites (-2 .. alphabet size), and the
+  // entries observed as 0 or an edit per problem (phi gradient); one
+  // branch-free pass over each row
   D.prob_nobs_phi.assign(P, 0.0);
   for (int v = 0; v < n; ++v) {
     const int t = d->tree_of[v];
     const int K = d->tree_nsites[t];
-    double c = 0.0;
-    for (int k = 0; k < K; ++k) c += d->leaf_codes[row_off[v] + k] >= 0 ? 1.0 : 0.0;
-    D.prob_nobs_phi[d->tree_problem[t]] += c;
+    const int16_t* c = d->leaf_codes + row_off[v];
+    const int32_t* m = alpha_size.data() + tree_site_base[t];
+    int bad = 0, obs = 0;
+    for (int k = 0; k < K; ++k) {
+      const int x = c[k];
+      bad |= (x < -2) | (x > m[k]);
+      obs += x >= 0;
+    }
+    if (bad) return fail(LAML_EINVAL, "leaf code out of range for its site");
+    D.prob_nobs_phi[d->tree_problem[t]] += (double)obs;
   }
   // nodes grouped by problem
   prob_node_off.assign(P + 1, 0);
@@ -384,11 +387,6 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
       row_off[v] = int_rows;
       int_rows += stride[v];
     }
-  }
-  std::vector<int16_t> codes_p((size_t)std::max<int64_t>(leaf_rows, 1), 0);
-  for (int v = 0; v < nl; ++v) {
-    const int K = d->tree_nsites[d->tree_of[v]];
-    std::memcpy(codes_p.data() + row_off[v], d->leaf_codes + D.row_off[v], sizeof(int16_t) * K);
   }
   int ndev = 0;
   LAML_HIP(hipGetDeviceCount(&ndev));
@@ -551,8 +549,16 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
     std::copy(d->prior, d->prior + d->site_qoff[S], pr.begin());
     UP(prior, pr.data(), pr.size());
   }
-  UP(codes, codes_p.data(), leaf_rows);
 #undef UP
+  {
+    // padded leaf rows; filled from the caller's packed codes below
+    int16_t* codes_dev = nullptr;
+    if ((rc = f->alloc(&codes_dev, (size_t)leaf_rows))) {
+      delete f;
+      return rc;
+    }
+    F.codes = codes_dev;
+  }
   {
     const int32_t *bt = nullptr, *bc = nullptr, *lbase = nullptr, *lbeg = nullptr, *lcnt = nullptr;
     if ((rc = f->upload(&bt, tw_blk_tree.data(), tw_blk_tree.size())) ||
@@ -692,6 +698,34 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
       hipMemset(nu_state, 0, sizeof(laml::NuState) * P) != hipSuccess) {
     delete f;
     return fail(LAML_EDEVICE, "hipMemset failed");
+  }
+  {
+    // The caller's codes go up as they are (rows of K sites back to back)
+    // with their row offsets, staged in the inside-row buffer (unused until
+    // the first pass, and 8x the leaf codes' bytes or more in a binary
+    // tree), and a kernel writes them into the padded rows.  The staging
+    // bytes are cleared afterwards (every pass writes an inside row before
+    // reading it; the clear keeps code bytes out of the rows' pad sites).
+    const size_t off_bytes = sizeof(int64_t) * (size_t)nl;
+    const size_t need = off_bytes + sizeof(int16_t) * (size_t)D.leaf_rows;
+    char* stage = static_cast<char*>(f->beta_alloc);
+    if (sizeof(double2) * (size_t)std::max<int64_t>(int_rows, 1) < need) {
+      if ((rc = f->alloc(&stage, need))) {
+        delete f;
+        return rc;
+      }
+    }
+    if (hipMemcpy(stage, D.row_off.data(), off_bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(stage + off_bytes, d->leaf_codes, need - off_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      delete f;
+      return fail(LAML_EDEVICE, "leaf code upload failed");
+    }
+    laml::run_pad_codes(F, reinterpret_cast<const int16_t*>(stage + off_bytes),
+                        reinterpret_cast<const int64_t*>(stage), nullptr);
+    if (hipMemsetAsync(stage, 0, need, nullptr) != hipSuccess) {
+      delete f;
+      return fail(LAML_EDEVICE, "hipMemsetAsync failed");
+    }
   }
   laml::run_labels(F, f->level_off, f->W, nullptr);
   hipError_t e = hipDeviceSynchronize();

@@ -67,6 +67,23 @@ __global__ void __launch_bounds__(256) k_labels(DevForest F, int beg, int cnt) {
   F.labels[F.row_off[v] + k] = (int16_t)combine_label(la, lb);
 }
 
+// Leaf code rows from the caller's packed layout (leaf v's K codes at
+// packed_off[v], rows back to back) into the device rows, padded to a
+// multiple of the segment width with code 0; a wavefront per row.
+__global__ void __launch_bounds__(256) k_pad_codes(DevForest F, const int16_t* __restrict__ packed,
+                                                   const int64_t* __restrict__ packed_off) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int W = F.chunk_w;
+  int16_t* codes = const_cast<int16_t*>(F.codes);
+  for (int v = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); v < F.num_leaves; v += (int)gridDim.x * 4) {
+    const int K = F.tree_nsites[F.tree_of[v]];
+    const int stride = (K + W - 1) / W * W;
+    const int16_t* src = packed + packed_off[v];
+    int16_t* dst = codes + F.row_off[v];
+    for (int k = lane; k < stride; k += 64) dst[k] = k < K ? src[k] : (int16_t)0;
+  }
+}
+
 // load the inside state of node c at site k (labels/codes by row_off, the
 // inside vector by brow, which differs from row_off under level checkpointing)
 __device__ __forceinline__ void load_child(const DevForest& F, int c, bool leaf, int k, double phi, int& lab,
@@ -2685,6 +2702,11 @@ void run_labels(const DevForest& F, const std::vector<int32_t>& lvl, int W, hipS
       hipLaunchKernelGGL(k_labels<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt);
     });
   }
+}
+
+void run_pad_codes(const DevForest& F, const int16_t* packed, const int64_t* packed_off, hipStream_t s) {
+  const int grid = std::max(1, std::min((F.num_leaves + 3) / 4, 8192));
+  hipLaunchKernelGGL(k_pad_codes, dim3(grid), dim3(256), 0, s, F, packed, packed_off);
 }
 
 void run_coef(const DevForest& F, const double* d, const double* nu, const int32_t* done, hipStream_t s) {

@@ -149,6 +149,9 @@ struct EmState {
 
 int seg_width(int max_k);
 void run_labels(const DevForest& F, const std::vector<int32_t>& lvl, int W, hipStream_t s);
+// leaf code rows from the packed host layout (uploaded as it is) into the
+// padded device rows of F.codes
+void run_pad_codes(const DevForest& F, const int16_t* packed, const int64_t* packed_off, hipStream_t s);
 void run_coef(const DevForest& F, const double* d, const double* nu, const int32_t* done, hipStream_t s);
 // wavefronts a row-kernel launch aims for before it gives each wavefront
 // more than one chunk of 64 sites (8 per SIMD of the 256 CUs)

@@ -256,15 +256,16 @@ class ForestLayout:
         self.level_off = np.searchsorted(hs, np.arange(H + 1)).astype(np.int32)
         self.num_leaves = int(self.level_off[1])
         self.tree_root = np.asarray([inv[base[i] + t.n_nodes - 1] for i, t in enumerate(trees)], dtype=np.int32)
-        leaf_glob = order[: self.num_leaves]
-        leaf_tree = tree_of_g[leaf_glob]
-        leaf_post = post_idx[leaf_glob]
-        code_chunks = []
-        for i, t in enumerate(trees):
-            sel = leaf_tree == i
-            rank = np.searchsorted(t.leaf_post, leaf_post[sel])
-            code_chunks.append(np.ascontiguousarray(t.codes[rank]).reshape(-1))
-        self.codes = np.concatenate(code_chunks).astype(np.int16) if code_chunks else np.zeros(0, np.int16)
+        # leaves (height 0) lead the level order, sorted by tree and then by
+        # post-order index, which is the order of each tree's code rows: the
+        # leaf codes are the trees' code matrices back to back (no gather;
+        # one tree's matrix is shared as a read-only view)
+        code_chunks = [np.ascontiguousarray(t.codes, dtype=np.int16).reshape(-1) for t in trees]
+        if len(code_chunks) == 1:
+            self.codes = code_chunks[0].view()
+            self.codes.flags.writeable = False
+        else:
+            self.codes = np.concatenate(code_chunks) if code_chunks else np.zeros(0, np.int16)
         qoff_parts = []
         prior_parts = []
         qsum_parts = []

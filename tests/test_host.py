@@ -124,6 +124,7 @@ def test_validation_rejects_bad_description():
     with pytest.raises(_lib.LamlError):
         lay.validate()
     lay, _, _ = _layout_for(T, msa, Q)
+    lay.codes = lay.codes.copy()
     lay.codes[3] = 999
     with pytest.raises(_lib.LamlError, match="leaf code"):
         lay.validate()
