@@ -91,6 +91,10 @@ struct laml_forest {
   // top of the chain region of each pass (levels below min(top, split));
   // 0: up to the split
   int chain_top_in = 5, chain_top_out = 10;
+  // steps per outside pack (chains of one tree run back to back by one
+  // wavefront; 1: a chain per wavefront)
+  int chain_pack = 8;
+  int64_t pack_waves = laml::kPackWaves;
   bool alpha_complete = true;
   std::deque<laml::ChainPlan> chain_plans;  // a deque: plans stay put when more are added
   std::vector<int32_t> h_heavy;  // heavy child of each node (-1: no internal child)
@@ -358,7 +362,6 @@ int laml_forest_validate(const laml_forest_desc* d) {
 static int upload_node_recs(laml_forest* f, const std::vector<int64_t>& brow);
 
 int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out) {
-  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
   if (!out) return fail(LAML_EINVAL, "null output pointer");
   *out = nullptr;
   Derived D;
@@ -388,6 +391,9 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
       int_rows += stride[v];
     }
   }
+  // host-only work above (validation, row layout) runs outside the lock;
+  // the device calls below are serialised with graph capture
+  std::lock_guard<std::mutex> legacy_lock(g_legacy_mu);
   int ndev = 0;
   LAML_HIP(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(LAML_EINVAL, "device out of range");
@@ -649,6 +655,10 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
     // default: outside chains below level 10 for forests with enough
     // (internal node, 64-site chunk) pairs to keep the chain rounds busy
     // (32k; smaller forests are latency-bound and keep level launches)
+    const char* penv = getenv("LAML_CHAIN_PACK");
+    if (penv) f->chain_pack = std::max(1, atoi(penv));
+    const char* wenv = getenv("LAML_PACK_WAVES");
+    if (wenv) f->pack_waves = std::max(1, atoi(wenv));
     const char* env = getenv("LAML_CHAINS");
     const int64_t units = (int64_t)(N - d->num_leaves) * ((max_k + 63) / 64);
     f->chains = env ? (atoi(env) & (LAML_CHAINS_OUTSIDE | LAML_CHAINS_INSIDE))
@@ -1125,7 +1135,7 @@ static int chains_now(const laml_forest* f) {
 // more than its parent's chain's (0 when the state comes from above).
 static const laml::ChainPlan* chain_plan(laml_forest* f, int hs) {
   for (const laml::ChainPlan& cp : f->chain_plans)
-    if (cp.h_split == hs) return &cp;
+    if (cp.h_split == hs && cp.pack_cap == f->chain_pack) return &cp;
   // a new plan is uploaded with synchronous copies: not while another
   // thread captures its EM step (the passes only look plans up: each entry
   // point builds them before any capture)
@@ -1159,20 +1169,21 @@ static const laml::ChainPlan* chain_plan(laml_forest* f, int hs) {
   }
   laml::ChainPlan cp;
   cp.h_split = hs;
+  cp.pack_cap = f->chain_pack;
   std::vector<int2> ch;
-  std::vector<int32_t> nodes, off_of(N, -1);
+  std::vector<int32_t> nodes, off_of(N, -1), round_off;
   std::vector<std::tuple<int, int, int>> in_heads;  // (inside round, -len, -head)
   ch.reserve(heads.size());
   nodes.reserve(B - lo);
   for (const auto& hd : heads) {
     const int r = std::get<0>(hd), v = -std::get<2>(hd);
-    while ((int)cp.round_off.size() <= r) cp.round_off.push_back((int32_t)ch.size());
+    while ((int)round_off.size() <= r) round_off.push_back((int32_t)ch.size());
     off_of[v] = (int32_t)nodes.size();
     ch.push_back(make_int2((int)nodes.size(), -std::get<1>(hd)));
     for (int w = v; w >= 0; w = f->h_heavy[w]) nodes.push_back(w);
     in_heads.emplace_back(in_round[v], std::get<1>(hd), std::get<2>(hd));
   }
-  cp.round_off.push_back((int32_t)ch.size());
+  round_off.push_back((int32_t)ch.size());
   std::sort(in_heads.begin(), in_heads.end());
   std::vector<int2> ich;
   ich.reserve(in_heads.size());
@@ -1194,8 +1205,71 @@ static const laml::ChainPlan* chain_plan(laml_forest* f, int hs) {
     r.ro0 = f->h_row_off[r.ch0];
     r.ro1 = f->h_row_off[r.ch1];
   }
-  if (f->upload(&cp.chains, ch.data(), ch.size()) || f->upload(&cp.in_chains, ich.data(), ich.size()) ||
-      f->upload(&cp.nodes, nodes.data(), nodes.size()) || f->upload(&cp.steps, steps.data(), steps.size()))
+  // Outside packs: per round, the chains of each tree (longest first) go
+  // back to back into packs of at most cap steps (a longer chain is a pack
+  // of its own), so that a wavefront's staging runs on from one chain into
+  // the next; the cap shrinks to keep kPackWaves wavefronts in the round,
+  // and packs are launched longest first.
+  std::vector<int2> packs;
+  std::vector<laml::ChainStep> psteps;
+  std::vector<int64_t> phro;
+  psteps.reserve(steps.size());
+  phro.reserve(steps.size());
+  {
+    struct Pack {
+      int len;
+      std::vector<int32_t> members;
+    };
+    for (size_t r = 0; r + 1 < round_off.size(); ++r) {
+      cp.pack_round_off.push_back((int32_t)packs.size());
+      std::vector<int32_t> order;
+      int64_t round_steps = 0;
+      for (int i = round_off[r]; i < round_off[r + 1]; ++i) {
+        order.push_back(i);
+        round_steps += ch[i].y;
+      }
+      const int cap = (int)std::max<int64_t>(
+          1, std::min<int64_t>(cp.pack_cap, round_steps * f->F.max_chunks / f->pack_waves));
+      // by tree, keeping the longest-first order within a tree
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return f->h_tree_of[nodes[ch[a].x]] < f->h_tree_of[nodes[ch[b].x]];
+      });
+      std::vector<Pack> rp;
+      int cur_tree = -1;
+      for (const int i : order) {
+        const int tr = f->h_tree_of[nodes[ch[i].x]];
+        if (tr != cur_tree || rp.back().len + ch[i].y > cap) {
+          rp.push_back(Pack{0, {}});
+          cur_tree = tr;
+        }
+        rp.back().len += ch[i].y;
+        rp.back().members.push_back(i);
+      }
+      std::stable_sort(rp.begin(), rp.end(), [](const Pack& a, const Pack& b) { return a.len > b.len; });
+      bool packed = false;
+      for (const Pack& pk : rp) packed = packed || pk.members.size() > 1;
+      cp.round_packed.push_back(packed ? 1 : 0);
+      for (const Pack& pk : rp) {
+        packs.push_back(make_int2((int)psteps.size(), pk.len));
+        for (const int i : pk.members) {
+          for (int j = 0; j < ch[i].y; ++j) {
+            laml::ChainStep rec = steps[ch[i].x + j];
+            if (j == 0) {
+              phro.push_back(f->h_row_off[rec.u]);
+              rec.u = ~rec.u;
+            } else {
+              phro.push_back(0);
+            }
+            psteps.push_back(rec);
+          }
+        }
+      }
+    }
+    cp.pack_round_off.push_back((int32_t)packs.size());
+  }
+  if (f->upload(&cp.in_chains, ich.data(), ich.size()) || f->upload(&cp.nodes, nodes.data(), nodes.size()) ||
+      f->upload(&cp.packs, packs.data(), packs.size()) || f->upload(&cp.pack_steps, psteps.data(), psteps.size()) ||
+      f->upload(&cp.pack_head_ro, phro.data(), phro.size()))
     return nullptr;
   f->chain_plans.push_back(std::move(cp));
   return &f->chain_plans.back();

@@ -1252,7 +1252,10 @@ constexpr int kChainLdsBE = 2048;      // [2][64] int32
 constexpr int kChainLdsQ = 2560;       // [2][64] 16 bytes
 constexpr int kChainLdsCoef = 4608;    // [64] uint32 (20 used)
 constexpr int kChainLdsCode = 4864;    // [2 slots][2 children][64] uint32
-constexpr int kChainLdsBytes = 5888;
+constexpr int kChainLdsHeadA = 5888;   // [2 halves][64] 16 bytes: a head's outside quad
+constexpr int kChainLdsHeadE = 7936;   // [64] int32: its exponent
+constexpr int kChainLdsHeadG = 8192;   // [64] uint32: the word holding its specific state
+constexpr int kChainLdsBytes = 8448;
 
 // copy `size` (4 or 16) bytes at src (per lane) to dst + size * lane; rows
 // read once per pass carry the non-temporal hint, so that the per-site
@@ -1336,6 +1339,16 @@ __device__ __forceinline__ void chain_stage(const DevForest& F, const ChainStep&
   }
 }
 
+// Copy the stored outside state (quad, exponent, specific state) of the node
+// whose rows start at ru: the head of the next chain of a pack.
+__device__ __forceinline__ void chain_stage_head(const DevForest& F, int64_t ru, unsigned kk, char* lds) {
+  const char* a = reinterpret_cast<const char*>(&row_at(F.alpha + ru, kk));
+  to_lds(a, lds + kChainLdsHeadA, 16, true);
+  to_lds(a + 16, lds + kChainLdsHeadA + 1024, 16, true);
+  to_lds(&row_at(F.aE + ru, kk), lds + kChainLdsHeadE, 4, true);
+  to_lds(F.aG + ru + (kk & ~1u), lds + kChainLdsHeadG, 4, true);
+}
+
 // One chain step at one site, on registers: the children's raw row values
 // (leaf codes or labels), gathered priors (entry for labels that name no
 // edit state: ignored), inside rows (ignored for leaves) and branch
@@ -1393,25 +1406,33 @@ __device__ __forceinline__ void chain_compute(const DevForest& F, const char* ld
             });
 }
 
-// Chains [cbeg, cbeg + cnt) of a chain plan: chain c's steps are
-// steps[chains[c].x .. chains[c].x + chains[c].y), head first.  Wavefront w
-// runs chain w / max_chunks at chunk w % max_chunks, so that every chunk of
-// the (longest, first) chains starts first.  Lanes past the tree's sites
+// Packs [cbeg, cbeg + cnt) of a chain plan: pack c's steps are
+// steps[packs[c].x .. packs[c].x + packs[c].y), one or more chains of one
+// tree back to back, each head first; a head's node id is stored
+// complemented (~u) and head_ro holds its row offset.  Wavefront w runs pack
+// w / max_chunks at chunk w % max_chunks, so that every chunk of the
+// (longest, first) packs starts first.  The first head's outside state is
+// loaded in the prologue; a later head's is staged into the slice during the
+// step before it, like every other operand.  Lanes past the tree's sites
 // read its last site (every address stays inside the rows) and neither
 // store nor add to the counts.  Grid: ceil(cnt * max_chunks / 4) blocks of
-// 256; the head's outside row must be complete.
-__global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* __restrict__ chains, int cbeg,
+// 256; the heads' outside rows must be complete.
+template <bool kPacked>
+__global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* __restrict__ packs, int cbeg,
                                                         int cnt, const ChainStep* __restrict__ steps,
+                                                        const int64_t* __restrict__ head_ro,
                                                         const int32_t* __restrict__ done) {
   __shared__ __align__(16) char lds_all[4 * kChainLdsBytes];
   const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   const int wv = w / F.max_chunks;
   if (wv >= cnt) return;
-  const int2 cd = uniform_load(chains + cbeg + wv);
+  const int2 cd = uniform_load(packs + cbeg + wv);
   const int nb = cd.x, len = cd.y;
   const ChainStep* cs = steps + nb;
+  const int64_t* hro = head_ro + nb;
   ChainStep r0 = uniform_load(cs);
-  const int t = F.tree_of[r0.u];
+  const int u0 = r0.u < 0 ? ~r0.u : r0.u;
+  const int t = F.tree_of[u0];
   const int p = F.tree_prob[t];
   const int K = F.tree_nsites[t];
   const int chunk = w - wv * F.max_chunks;
@@ -1429,7 +1450,7 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
   const int LE = row_at(F.LE + sb, kk);
   const int qo = row_at(F.site_qoff + sb, kk) - 1;
   const double qs = row_at(F.site_qsum + sb, kk);
-  const int64_t ru = F.row_off[r0.u];
+  const int64_t ru = F.row_off[u0];
   double4 au = row_at(F.alpha + ru, kk);
   int Eo = row_at(F.aE + ru, kk);
   int gu = row_at(F.aG + ru, kk);
@@ -1457,7 +1478,19 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
     r0 = uniform_load(cs + j);
     const ChainStep r1 = uniform_load(cs + (more ? j + 1 : j));
     const ChainStep r2 = uniform_load(cs + (more2 ? j + 2 : j));
+    const bool head1 = kPacked && more && r1.u < 0;
+    const int64_t ru1 = head1 ? uniform_load(hro + j + 1) : 0;
     if (j + 4 < len) touch = uniform_load(&cs[j + 4].u);
+    // a later chain's head: its outside state, staged during the last step
+    const bool head0 = r0.u < 0;
+    const int u = head0 ? ~r0.u : r0.u;
+    if (kPacked && j > 0 && head0) {
+      const double2 xy = lds_at<double2>(lds, kChainLdsHeadA + 16 * lane);
+      const double2 zw = lds_at<double2>(lds, kChainLdsHeadA + 1024 + 16 * lane);
+      au = make_double4(xy.x, xy.y, zw.x, zw.y);
+      Eo = lds_at<int>(lds, kChainLdsHeadE + 4 * lane);
+      gu = code_of_word(lds_at<uint32_t>(lds, kChainLdsHeadG + 4 * lane), kk);
+    }
     // this step's operands and the next step's codes, from the slice
     // (wave-uniform: moved to scalar registers)
     Coef cf[2];
@@ -1481,6 +1514,7 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
       if (more) {
         chain_stage(F, r1, rawn, qo, kk, lane, lds);
         if (more2) chain_stage_codes(F, r2, kk, slot, lds);
+        if (head1) chain_stage_head(F, ru1, kk, lds);
       }
     };
     // step j
@@ -1525,7 +1559,7 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
         const int c = vi >= 5 ? 1 : 0;
         pp = F.cpart + ((int64_t)(c ? r0.ch1 : r0.ch0) * F.max_chunks + chunk) * 5 + (vi - 5 * c);
       } else if (i < 3 && vi == 10) {
-        pp = F.rt_part + (int64_t)r0.u * F.max_chunks + chunk;
+        pp = F.rt_part + (int64_t)u * F.max_chunks + chunk;
       }
     }
     if (!more) break;
@@ -2869,12 +2903,16 @@ void run_inside_chains(const DevForest& F, const ChainPlan& cp, const int32_t* d
 }
 
 void run_outside_chains(const DevForest& F, const ChainPlan& cp, const int32_t* done, hipStream_t s) {
-  for (size_t r = 0; r + 1 < cp.round_off.size(); ++r) {
-    const int beg = cp.round_off[r], cnt = cp.round_off[r + 1] - beg;
+  for (size_t r = 0; r + 1 < cp.pack_round_off.size(); ++r) {
+    const int beg = cp.pack_round_off[r], cnt = cp.pack_round_off[r + 1] - beg;
     if (cnt <= 0) continue;
     const int64_t waves = (int64_t)cnt * F.max_chunks;
-    hipLaunchKernelGGL(k_outside_chain, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, F, cp.chains, beg, cnt,
-                       cp.steps, done);
+    if (cp.round_packed[r])
+      hipLaunchKernelGGL(k_outside_chain<true>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, F, cp.packs, beg,
+                         cnt, cp.pack_steps, cp.pack_head_ro, done);
+    else
+      hipLaunchKernelGGL(k_outside_chain<false>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, F, cp.packs, beg,
+                         cnt, cp.pack_steps, cp.pack_head_ro, done);
   }
 }
 

@@ -168,15 +168,15 @@ struct CherrySplit {
 };
 
 // Heavy-path chains over the levels below h_split (k_outside_chain,
-// k_inside_chain): chain c is nodes[chains[c].x .. chains[c].x +
-// chains[c].y), head first, each later node the heavy child of the one
-// before.  Outside round r holds chains [round_off[r], round_off[r + 1]),
-// longest first; the heads of round 0 get their outside state from the
-// levels above h_split or the root step, those of round r > 0 from round
-// r - 1.  in_chains lists the same chains in inside rounds: a chain runs
-// one round after the last chain hanging off it (its light internal
-// children's), round 0 holding chains with none (device arrays, host
-// offsets).
+// k_inside_chain): a chain is a node and its successive heavy children,
+// head first.  Outside rounds: the heads of round 0 get their outside state
+// from the levels above h_split or the root step, those of round r > 0 from
+// round r - 1; round r's packs are [pack_round_off[r], pack_round_off[r +
+// 1]), longest first, each the steps of one or more of the round's chains of
+// one tree.  in_chains lists the chains (offset into nodes, length) in
+// inside rounds: a chain runs one round after the last chain hanging off it
+// (its light internal children's), round 0 holding chains with none (device
+// arrays, host offsets).
 // One node of a chain, as the outside chain kernel reads it: the node, its
 // children, which child continues the chain (-1: none), and the children's
 // row offsets.
@@ -187,12 +187,22 @@ struct ChainStep {
 
 struct ChainPlan {
   int32_t h_split = -1;
-  const int2* chains = nullptr;
+  int32_t pack_cap = 0;              // most steps per outside pack the plan was built for
   const int2* in_chains = nullptr;
   const int32_t* nodes = nullptr;
-  const ChainStep* steps = nullptr;  // [nodes] the record of each entry of nodes
-  std::vector<int32_t> round_off, in_round_off;
+  // outside packs: (offset into pack_steps, steps), each one or more chains
+  // of one tree back to back; a chain's first record holds ~u and
+  // pack_head_ro its row offset
+  const int2* packs = nullptr;
+  const ChainStep* pack_steps = nullptr;
+  const int64_t* pack_head_ro = nullptr;
+  std::vector<int32_t> pack_round_off, in_round_off;
+  std::vector<uint8_t> round_packed;  // per round: some pack holds more than one chain
 };
+// wavefronts an outside chain round keeps: its packs hold at most
+// max(1, min(cap, (steps x chunks) / kPackWaves)) steps, so that packing
+// only thins rounds with wavefronts to spare (16 per SIMD)
+constexpr int64_t kPackWaves = 16384;
 
 // cp: the levels below the top kernel's run as heavy-path chains
 void run_inside(const DevForest& F, const std::vector<int32_t>& lvl, int W, int h_top, const std::vector<int32_t>& cpw,

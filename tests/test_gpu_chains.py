@@ -357,3 +357,50 @@ def test_em_batched_problems_equal_each_alone(nprob):
         n1, i1, l1, nu1, phi1 = run([p])
         assert nll[i] == n1[0] and it[i] == i1[0] and nu[i] == nu1[0] and phi[i] == phi1[0], i
         np.testing.assert_array_equal(lens[i], l1[0])
+
+
+@pytest.mark.parametrize("cap", [2, 3, 8, 40])
+def test_outside_packs_equal_one_chain_per_wavefront(cap, monkeypatch):
+    """Outside packs (a wavefront runs several chains of one tree back to
+    back, each later head's outside state staged during the step before it)
+    at several pack sizes give the bits of one chain per wavefront and of
+    level passes: E-step counts, dropout sums, likelihoods and an EM run of
+    8 steps, on four trees of three problems with ragged site counts."""
+    from laml_amd.forest import ProblemSpec
+
+    rng = np.random.Generator(np.random.PCG64(2207))
+    q = sim.random_priors(130, 3, 20, 1.0, rng)
+    trees = [sim.yule_tree(n, rng) for n in (900, 380)]
+    codes = [sim.simulate_codes(t, q, 0.2, 0.3, rng) for t in trees]
+    parts = [sim.problem_spec(t, q, c) for t, c in zip(trees, codes)]
+    multi = ProblemSpec([p.trees[0] for p in parts], parts[0].sites, sum(p.nobs for p in parts),
+                        min(p.phimin for p in parts))
+    probs = [_problem("yule", 700, 75, 2208), _problem("caterpillar", 300, 140, 2209)]
+    specs = [multi] + [sim.problem_spec(t, q2, c) for (t, q2, c, _) in probs]
+    lengths = [t.length for t in trees] + [t.length for (t, _, _, _) in probs]
+    dev = torch.cuda.current_device()
+    res = []
+    # pack every round up to the cap (small forests keep one chain per
+    # wavefront by default)
+    monkeypatch.setenv("LAML_PACK_WAVES", "1")
+    for pack, on in ((cap, True), (1, True), (1, False)):
+        monkeypatch.setenv("LAML_CHAIN_PACK", str(pack))
+        print("outside packs", pack, on, flush=True)
+        F = Forest(specs, device=dev)
+        F.set_pass_mode(0)
+        H = F.num_levels
+        F.set_shapes(np.ones(H, dtype=np.int32), np.ones(H, dtype=np.int32), H)
+        F.set_chains(1 if on else False, 0, 0)
+        assert F.chains() == (1 if on else 0)
+        d = torch.tensor(F.lengths_to_level(lengths), dtype=torch.float64, device=F.device)
+        nu = torch.tensor([0.2, 0.1, 0.3], dtype=torch.float64, device=F.device)
+        phi = torch.tensor([0.3, 0.15, 0.1], dtype=torch.float64, device=F.device)
+        counts, rt, nll = F.estep(d, nu, phi)
+        e = (counts.cpu().numpy().copy(), rt.cpu().numpy().copy(), nll.cpu().numpy().copy())
+        nll2, _ = F.em(d, nu, phi, 8, 0.0)
+        res.append(e + (nll2.cpu().numpy().copy(), d.cpu().numpy().copy(), nu.cpu().numpy().copy(),
+                        phi.cpu().numpy().copy()))
+        F.close()
+    for other in res[1:]:
+        for a, b in zip(res[0], other):
+            np.testing.assert_array_equal(a, b)
