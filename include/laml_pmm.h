@@ -173,7 +173,10 @@ int32_t laml_forest_fusion(laml_forest* f);
  * reads only the other children's outside rows; LAML_CHAINS_INSIDE, the
  * inside pass walks them bottom-up and reads only the other children's
  * inside rows (every inside row is still written).  Chains run in rounds
- * (one launch each) ordered by their dependences.  Results are
+ * (one launch each) ordered by their dependences; in an outside round with
+ * wavefronts to spare one wavefront runs several short chains of one tree
+ * back to back, up to 8 steps (environment LAML_CHAIN_PACK sets that cap at
+ * forest creation, 1: one chain per wavefront).  Results are
  * bit-identical to level launches with one chunk per wavefront.  After an
  * outside pass with chains the heavy children's outside rows are not
  * stored: laml_posteriors and laml_quartet_em then recompute every outside
