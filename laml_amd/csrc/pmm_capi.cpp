@@ -738,7 +738,10 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
     }
   }
   laml::run_labels(F, f->level_off, f->W, nullptr);
-  hipError_t e = hipDeviceSynchronize();
+  // every set-up copy and kernel above ran on the legacy default stream:
+  // waiting for it alone leaves other threads' streams (a server's other
+  // solves) running
+  hipError_t e = hipStreamSynchronize(nullptr);
   if (e != hipSuccess) {
     delete f;
     return fail(LAML_EDEVICE, std::string("label kernels: ") + hipGetErrorString(e));
