@@ -925,9 +925,11 @@ def main():
                      "traffic_gbs": round(traffic / (out_ms * 1e-3) / 1e9, 1) if traffic else None,
                      "note": "achieved counts the compulsory bytes of the algorithm the plan runs (outside chains "
                              "keep heavy children's outside states in registers, about half the bytes of level "
-                             "launches); the chain steps are bound by instruction issue (FP64 VALU, prior "
-                             "gathers that touch a cache line per lane) and memory latency together, not by HBM "
-                             "bandwidth (traffic_gbs)",
+                             "launches); the chain steps read each child's prior weight from a row built at "
+                             "forest creation (8 B per child and site, streamed) instead of gathering prior-table "
+                             "entries, a line per lane: those bytes are in traffic, not in achieved; the chain "
+                             "steps are bound by instruction issue (FP64 VALU) and memory latency together, not by "
+                             "HBM bandwidth (traffic_gbs)",
                      "launch_plan": "as in " + PROFILE if roof_plan is not None else "as the batch",
                      "kernel": "outside pass (laml_outside: k_tree_walk top levels + root in the hybrid plan, "
                                "k_outside_chain rounds below the split when the plan has chains, else "

@@ -593,6 +593,13 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   double *iLm, *site_llh, *rt_part, *cpart, *nu_a, *nu_b, *nu_d, *nu_part;
   laml::NuState* nu_state;
   AL(labels, int16_t, int_rows);
+  // prior-weight rows: read by the chain kernels only, which need 64-site
+  // segments
+  double *qleaf = nullptr, *qint = nullptr;
+  if (f->W == 64) {
+    AL(qleaf, double, leaf_rows);
+    AL(qint, double, int_rows);
+  }
   AL(coef, laml::Coef, N);
   {
     void* q = nullptr;
@@ -632,6 +639,8 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
   AL(f->em.pre, double, P);
 #undef AL
   F.labels = labels;
+  F.qleaf = qleaf;
+  F.qint = qint;
   F.brow = F.row_off;
   f->h_parent.assign(d->parent, d->parent + N);
   f->h_tree_of.assign(d->tree_of, d->tree_of + N);
@@ -738,6 +747,7 @@ int laml_forest_create(const laml_forest_desc* d, int device, laml_forest** out)
     }
   }
   laml::run_labels(F, f->level_off, f->W, nullptr);
+  if (qleaf) laml::run_qrows(F, qleaf, qint, nullptr);
   // every set-up copy and kernel above ran on the legacy default stream:
   // waiting for it alone leaves other threads' streams (a server's other
   // solves) running

@@ -67,6 +67,30 @@ __global__ void __launch_bounds__(256) k_labels(DevForest F, int beg, int cnt) {
   F.labels[F.row_off[v] + k] = (int16_t)combine_label(la, lb);
 }
 
+// Prior-weight rows of every node (a wavefront per row; sites past the
+// tree's up to the row stride get 0).
+__global__ void __launch_bounds__(256) k_qrows(DevForest F, double* __restrict__ qleaf, double* __restrict__ qint) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int W = F.chunk_w;
+  for (int v = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); v < F.num_nodes; v += (int)gridDim.x * 4) {
+    const int t = F.tree_of[v];
+    const int K = F.tree_nsites[t];
+    const int stride = (K + W - 1) / W * W;
+    const int sb = F.tree_site_base[t];
+    const int64_t ro = F.row_off[v];
+    const bool leaf = v < F.num_leaves;
+    double* dst = (leaf ? qleaf : qint) + ro;
+    for (int k = lane; k < stride; k += 64) {
+      double q = 0.0;
+      if (k < K) {
+        const int lab = leaf ? code_label(F.codes[ro + k]) : (int)F.labels[ro + k];
+        q = lab > 0 ? F.prior[F.site_qoff[sb + k] - 1 + lab] : F.site_qsum[sb + k];
+      }
+      dst[k] = q;
+    }
+  }
+}
+
 // Leaf code rows from the caller's packed layout (leaf v's K codes at
 // packed_off[v], rows back to back) into the device rows, padded to a
 // multiple of the segment width with code 0; a wavefront per row.
@@ -1237,6 +1261,9 @@ __global__ void __launch_bounds__(256) k_outside_rows_ch(DevForest F, int beg, i
 __device__ __forceinline__ const int16_t* chain_code_row(const DevForest& F, int c, int64_t ro) {
   return (c < F.num_leaves ? F.codes : F.labels) + ro;
 }
+__device__ __forceinline__ const double* chain_q_row(const DevForest& F, int c, int64_t ro) {
+  return (c < F.num_leaves ? F.qleaf : F.qint) + ro;
+}
 
 typedef __attribute__((address_space(3))) void lds_void;
 typedef const __attribute__((address_space(1))) void glb_void;
@@ -1249,7 +1276,7 @@ typedef const __attribute__((address_space(1))) void glb_void;
 // steps.
 constexpr int kChainLdsBeta = 0;       // [2][64] double2
 constexpr int kChainLdsBE = 2048;      // [2][64] int32
-constexpr int kChainLdsQ = 2560;       // [2][64] 16 bytes
+constexpr int kChainLdsQ = 2560;       // [2][64] double (1 KiB of the 2 KiB region used)
 constexpr int kChainLdsCoef = 4608;    // [64] uint32 (20 used)
 constexpr int kChainLdsCode = 4864;    // [2 slots][2 children][64] uint32
 constexpr int kChainLdsHeadA = 5888;   // [2 halves][64] 16 bytes: a head's outside quad
@@ -1316,10 +1343,11 @@ __device__ __forceinline__ void chain_stage_codes(const DevForest& F, const Chai
 }
 
 // Copy step r's operands: branch coefficients of both children, inside rows
-// of internal children, prior gathers of both from their raw codes (entry 0
-// for labels that name no edit state, so that every lane copies).
-__device__ __forceinline__ void chain_stage(const DevForest& F, const ChainStep& r, const int (&raw)[2], int qo,
-                                            unsigned kk, int lane, char* lds) {
+// of internal children, and both children's prior-weight rows over the
+// chunk's 64 sites (k0 .. k0 + 63, inside the padded rows; 16 bytes by each
+// of the first 32 lanes).
+__device__ __forceinline__ void chain_stage(const DevForest& F, const ChainStep& r, unsigned k0, unsigned kk,
+                                            int lane, char* lds) {
   {
     const int c = lane < 10 ? r.ch0 : r.ch1;
     const int wd = lane < 10 ? lane : min(lane - 10, 9);
@@ -1334,8 +1362,7 @@ __device__ __forceinline__ void chain_stage(const DevForest& F, const ChainStep&
       to_lds(&row_at(F.beta + ro[i], kk), lds + kChainLdsBeta + 1024 * i, 16, true);
       to_lds(&row_at(F.bE + ro[i], kk), lds + kChainLdsBE + 256 * i, 4, true);
     }
-    const int lab = leaf ? code_label(raw[i]) : raw[i];
-    to_lds(F.prior + (lab > 0 ? qo + lab : 0), lds + kChainLdsQ + 1024 * i, 16);
+    if (lane < 32) to_lds(chain_q_row(F, ch[i], ro[i]) + k0 + 2 * lane, lds + kChainLdsQ + 512 * i, 16, true);
   }
 }
 
@@ -1350,10 +1377,9 @@ __device__ __forceinline__ void chain_stage_head(const DevForest& F, int64_t ru,
 }
 
 // One chain step at one site, on registers: the children's raw row values
-// (leaf codes or labels), gathered priors (entry for labels that name no
-// edit state: ignored), inside rows (ignored for leaves) and branch
-// coefficients.  The heavy child's outside state replaces (au, Eo, gu); the
-// light internal child's is stored at row ro[light].
+// (leaf codes or labels), prior weights, inside rows (ignored for leaves)
+// and branch coefficients.  The heavy child's outside state replaces (au,
+// Eo, gu); the light internal child's is stored at row ro[light].
 template <int K0, int K1, typename Stage>
 __device__ __forceinline__ void chain_compute(const DevForest& F, const char* lds, int lane, const Coef (&cf)[2],
                                               double phi, int hv, double iL, int LE, double qs, const int (&raw)[2],
@@ -1365,7 +1391,7 @@ __device__ __forceinline__ void chain_compute(const DevForest& F, const char* ld
   double b0[2], bA[2], q[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
-    const double g = lds_at<double>(lds, kChainLdsQ + 1024 * i + 16 * lane);
+    q[i] = lds_at<double>(lds, kChainLdsQ + 512 * i + 8 * lane);
     if (lf[i]) {
       code[i] = raw[i];
       leaf_beta(raw[i], phi, lab[i], b0[i], bA[i]);
@@ -1378,7 +1404,6 @@ __device__ __forceinline__ void chain_compute(const DevForest& F, const char* ld
       bA[i] = bb.y;
       E[i] = lds_at<int>(lds, kChainLdsBE + 256 * i + 4 * lane);
     }
-    q[i] = lab[i] > 0 ? g : qs;
   }
   // the slice is read: the next step's copies may overwrite it
   chain_wait_lgkm();
@@ -1448,8 +1473,8 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
   const bool stream = F.stream_rows != 0;
   const double iL = row_at(F.iLm + sb, kk);
   const int LE = row_at(F.LE + sb, kk);
-  const int qo = row_at(F.site_qoff + sb, kk) - 1;
   const double qs = row_at(F.site_qsum + sb, kk);
+  const unsigned k0 = (unsigned)chunk * 64u;
   const int64_t ru = F.row_off[u0];
   double4 au = row_at(F.alpha + ru, kk);
   int Eo = row_at(F.aE + ru, kk);
@@ -1457,12 +1482,9 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
 
   // prologue: step 0's operands and codes and step 1's codes staged
   {
-    int raw0[2];
-    raw0[0] = row_at(chain_code_row(F, r0.ch0, r0.ro0), kk);
-    raw0[1] = row_at(chain_code_row(F, r0.ch1, r0.ro1), kk);
     chain_stage_codes(F, r0, kk, 0, lds);
     if (len > 1) chain_stage_codes(F, uniform_load(cs + 1), kk, 1, lds);
-    chain_stage(F, r0, raw0, qo, kk, lane, lds);
+    chain_stage(F, r0, k0, kk, lane, lds);
   }
   int touch = 0;  // first word of the record four steps ahead (brings its line into the scalar cache)
   if (len > 2) touch = uniform_load(&cs[min(len - 1, 4)].u);
@@ -1506,13 +1528,11 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
     const int slot = j & 1;
     const int raw[2] = {code_of_word(lds_at<uint32_t>(lds, kChainLdsCode + 512 * slot + 4 * lane), kk),
                         code_of_word(lds_at<uint32_t>(lds, kChainLdsCode + 512 * slot + 256 + 4 * lane), kk)};
-    const int rawn[2] = {code_of_word(lds_at<uint32_t>(lds, kChainLdsCode + 512 * (slot ^ 1) + 4 * lane), kk),
-                         code_of_word(lds_at<uint32_t>(lds, kChainLdsCode + 512 * (slot ^ 1) + 256 + 4 * lane), kk)};
     // operands of step j + 1 and codes of step j + 2 (into this step's
     // slot), issued once this step's operands are read from the slice
     auto stage = [&]() {
       if (more) {
-        chain_stage(F, r1, rawn, qo, kk, lane, lds);
+        chain_stage(F, r1, k0, kk, lane, lds);
         if (more2) chain_stage_codes(F, r2, kk, slot, lds);
         if (head1) chain_stage_head(F, ru1, kk, lds);
       }
@@ -2736,6 +2756,11 @@ void run_labels(const DevForest& F, const std::vector<int32_t>& lvl, int W, hipS
       hipLaunchKernelGGL(k_labels<WW>, seg_grid(cnt, WW, F.max_chunks), dim3(256), 0, s, F, beg, cnt);
     });
   }
+}
+
+void run_qrows(const DevForest& F, double* qleaf, double* qint, hipStream_t s) {
+  const int grid = std::max(1, std::min((F.num_nodes + 3) / 4, 8192));
+  hipLaunchKernelGGL(k_qrows, dim3(grid), dim3(256), 0, s, F, qleaf, qint);
 }
 
 void run_pad_codes(const DevForest& F, const int16_t* packed, const int64_t* packed_off, hipStream_t s) {

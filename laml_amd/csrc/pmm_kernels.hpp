@@ -76,6 +76,12 @@ struct DevForest {
   const double* site_qsum;   // prior mass of each site's alphabet (1 for normalised priors)
   const double* prior;
   const int16_t* codes;  // leaf rows
+  // prior weight of each node's label at each site (leaf rows by the leaf
+  // row offsets, internal rows by the internal ones): prior[site_qoff[s] - 1 +
+  // label] for an edit label, the site's prior mass site_qsum[s] otherwise;
+  // fixed for the forest's life (labels and priors are)
+  const double* qleaf;
+  const double* qint;
   // per-call parameters
   const double* phi;  // [P]
   // state
@@ -149,6 +155,8 @@ struct EmState {
 
 int seg_width(int max_k);
 void run_labels(const DevForest& F, const std::vector<int32_t>& lvl, int W, hipStream_t s);
+// the prior-weight rows qleaf / qint from the codes and labels (after run_labels)
+void run_qrows(const DevForest& F, double* qleaf, double* qint, hipStream_t s);
 // leaf code rows from the packed host layout (uploaded as it is) into the
 // padded device rows of F.codes
 void run_pad_codes(const DevForest& F, const int16_t* packed, const int64_t* packed_off, hipStream_t s);

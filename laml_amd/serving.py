@@ -56,8 +56,10 @@ def estimate_forest_bytes(specs):
         for t in p.trees:
             n_leaf = t.n_leaves
             n_int = t.n_nodes - n_leaf
-            # leaf codes 2 B; internal: beta 16 + bE 4 + labels 2 + alpha 32 + aE 4 + aG 2
-            total += n_leaf * kp * 2 + n_int * kp * 60
+            # leaf codes 2 B; internal: beta 16 + bE 4 + labels 2 + alpha 32 + aE 4 + aG 2;
+            # with 64-site segments every node also has a prior-weight row (8 B)
+            q = 8 if W == 64 else 0
+            total += n_leaf * kp * (2 + q) + n_int * kp * (60 + q)
             nodes += t.n_nodes
         total += p.sites.K * (8 + 4 + 8 + 4 + 4) + 8 * int(p.sites.qoff[-1])
     total += nodes * (4 * 8 + 8 * 3 + 32 + 48 * chunks + 8 * 8)
