@@ -1246,12 +1246,12 @@ __global__ void __launch_bounds__(256) k_outside_rows_ch(DevForest F, int beg, i
 //
 // The steps are software-pipelined so that each waits for memory once, on
 // operations issued a whole step earlier.  The inside rows, branch
-// coefficients and prior gathers of step j + 1 and the codes of step j + 2
-// are copied to the wavefront's slice of LDS during step j by direct
-// global-to-LDS loads, which occupy no registers in flight.  At the top of
-// step j every outstanding access completes, step j's operands and step
-// j + 1's codes are read from LDS and the step records from the scalar
-// cache, and the next copies are issued; step j itself runs on registers.
+// coefficients, prior-weight rows and codes of step j + 1 are copied to the
+// wavefront's slice of LDS during step j by direct global-to-LDS loads,
+// which occupy no registers in flight.  At the top of step j every
+// outstanding access completes, step j's operands are read from LDS and the
+// step records from the scalar cache, and the next copies are issued; step
+// j itself runs on registers.
 // No register written by a memory load is carried from one step to the
 // next, so the compiler never waits for one mid-step.  The vector memory
 // counter counts loads and stores together, so the partial counts of step j
@@ -1269,16 +1269,14 @@ typedef __attribute__((address_space(3))) void lds_void;
 typedef const __attribute__((address_space(1))) void glb_void;
 
 // One wavefront's LDS slice: the next step's inside rows of both children,
-// both children's prior gathers (16 bytes from the gathered entry, the
-// first 8 used: one access per lane and child), both children's branch
-// coefficients (10 words each), and two slots of codes (the aligned 32-bit
-// word holding each lane's 16-bit code, both children), used by alternate
-// steps.
+// both children's prior weights, both children's branch coefficients (10
+// words each), both children's codes (the aligned 32-bit word holding each
+// lane's 16-bit code), and in a pack the next head's outside state.
 constexpr int kChainLdsBeta = 0;       // [2][64] double2
 constexpr int kChainLdsBE = 2048;      // [2][64] int32
 constexpr int kChainLdsQ = 2560;       // [2][64] double (1 KiB of the 2 KiB region used)
 constexpr int kChainLdsCoef = 4608;    // [64] uint32 (20 used)
-constexpr int kChainLdsCode = 4864;    // [2 slots][2 children][64] uint32
+constexpr int kChainLdsCode = 4864;    // [2 children][64] uint32 (512 of 1024 bytes used)
 constexpr int kChainLdsHeadA = 5888;   // [2 halves][64] 16 bytes: a head's outside quad
 constexpr int kChainLdsHeadE = 7936;   // [64] int32: its exponent
 constexpr int kChainLdsHeadG = 8192;   // [64] uint32: the word holding its specific state
@@ -1333,7 +1331,7 @@ __device__ __forceinline__ int code_of_word(uint32_t wd, unsigned kk) {
   return (int)(int16_t)(kk & 1u ? wd >> 16 : wd & 0xffffu);
 }
 
-// Copy step r's codes into code slot `slot`.
+// Copy step r's codes into code slot `slot` (0).
 __device__ __forceinline__ void chain_stage_codes(const DevForest& F, const ChainStep& r, unsigned kk, int slot,
                                                   char* lds) {
   const int16_t* row0 = chain_code_row(F, r.ch0, r.ro0) + (kk & ~1u);
@@ -1483,7 +1481,6 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
   // prologue: step 0's operands and codes and step 1's codes staged
   {
     chain_stage_codes(F, r0, kk, 0, lds);
-    if (len > 1) chain_stage_codes(F, uniform_load(cs + 1), kk, 1, lds);
     chain_stage(F, r0, k0, kk, lane, lds);
   }
   int touch = 0;  // first word of the record four steps ahead (brings its line into the scalar cache)
@@ -1496,10 +1493,8 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
     asm volatile("" ::"s"(touch));
     if (pp) *pp = pv;
     const bool more = j + 1 < len;
-    const bool more2 = j + 2 < len;
     r0 = uniform_load(cs + j);
     const ChainStep r1 = uniform_load(cs + (more ? j + 1 : j));
-    const ChainStep r2 = uniform_load(cs + (more2 ? j + 2 : j));
     const bool head1 = kPacked && more && r1.u < 0;
     const int64_t ru1 = head1 ? uniform_load(hro + j + 1) : 0;
     if (j + 4 < len) touch = uniform_load(&cs[j + 4].u);
@@ -1513,8 +1508,8 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
       Eo = lds_at<int>(lds, kChainLdsHeadE + 4 * lane);
       gu = code_of_word(lds_at<uint32_t>(lds, kChainLdsHeadG + 4 * lane), kk);
     }
-    // this step's operands and the next step's codes, from the slice
-    // (wave-uniform: moved to scalar registers)
+    // this step's operands, from the slice (wave-uniform: moved to scalar
+    // registers)
     Coef cf[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -1525,15 +1520,14 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
       cf[i].esil = __builtin_amdgcn_readfirstlane(lds_at<int>(lds, kChainLdsCoef + 40 * i + 32));
       cf[i].pad = 0;
     }
-    const int slot = j & 1;
-    const int raw[2] = {code_of_word(lds_at<uint32_t>(lds, kChainLdsCode + 512 * slot + 4 * lane), kk),
-                        code_of_word(lds_at<uint32_t>(lds, kChainLdsCode + 512 * slot + 256 + 4 * lane), kk)};
-    // operands of step j + 1 and codes of step j + 2 (into this step's
-    // slot), issued once this step's operands are read from the slice
+    const int raw[2] = {code_of_word(lds_at<uint32_t>(lds, kChainLdsCode + 4 * lane), kk),
+                        code_of_word(lds_at<uint32_t>(lds, kChainLdsCode + 256 + 4 * lane), kk)};
+    // operands and codes of step j + 1, issued once this step's are read
+    // from the slice
     auto stage = [&]() {
       if (more) {
         chain_stage(F, r1, k0, kk, lane, lds);
-        if (more2) chain_stage_codes(F, r2, kk, slot, lds);
+        chain_stage_codes(F, r1, kk, 0, lds);
         if (head1) chain_stage_head(F, ru1, kk, lds);
       }
     };
