@@ -362,6 +362,16 @@ __device__ __forceinline__ void inside_site(const DevForest& F, const InNode& n,
   inside_gather(F, x);
   inside_finish(n, k, x);
 }
+// the same with the children's prior weights read from their prior-weight
+// rows q0, q1 (no load waits for the labels)
+__device__ __forceinline__ void inside_site_q(const DevForest& F, const InNode& n, int k, const double* q0,
+                                              const double* q1) {
+  InSite x;
+  inside_load<kNoCherry, kNoCherry>(F, n, k, x);
+  x.qa = row_at(q0, k);
+  x.qb = row_at(q1, k);
+  inside_finish(n, k, x);
+}
 
 // The inside-pass values of internal node v from its record (children that
 // are leaves or stored internal nodes).
@@ -1036,6 +1046,18 @@ __device__ __forceinline__ void outside_site(const DevForest& F, const OutNode& 
   outside_gather(F, x);
   outside_finish<K0, K1>(F, n, k, x, acc, rt, cacc, crt);
 }
+// the same with the children's prior weights read from their prior-weight
+// rows q0, q1, for nodes without fused cherry children
+__device__ __forceinline__ void outside_site_q(const DevForest& F, const OutNode& n, int k, double (&acc)[2][5],
+                                               double& rt, const double* q0, const double* q1) {
+  double cacc[2][2][5];
+  double crt[2];
+  OutSite x;
+  outside_load<kNoCherry, kNoCherry>(F, n, k, x);
+  x.q[0] = row_at(q0, k);
+  x.q[1] = row_at(q1, k);
+  outside_finish<kNoCherry, kNoCherry>(F, n, k, x, acc, rt, cacc, crt);
+}
 
 // outside_site for kernels that never meet a fused cherry
 template <int K0, int K1>
@@ -1639,8 +1661,12 @@ __global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWal
         const int v = W == 64 ? __builtin_amdgcn_readfirstlane(beg + i) : beg + i;
         if (k < K) {
           InNode n;
-          in_node_rec(F, uniform_load(F.nrec + (v - F.num_leaves)), n);
-          inside_site<kNoCherry, kNoCherry>(F, n, k);
+          const NodeRec r = uniform_load(F.nrec + (v - F.num_leaves));
+          in_node_rec(F, r, n);
+          if constexpr (W == 64)
+            inside_site_q(F, n, k, chain_q_row(F, r.ch0, r.ro0), chain_q_row(F, r.ch1, r.ro1));
+          else
+            inside_site<kNoCherry, kNoCherry>(F, n, k);
         }
       }
       __syncthreads();
@@ -1656,14 +1682,19 @@ __global__ void __launch_bounds__(kWalkThreads) k_tree_walk(DevForest F, TreeWal
       for (int i = seg; i < cnt; i += kSegs) {
         const int u = W == 64 ? __builtin_amdgcn_readfirstlane(beg + i) : beg + i;
         OutNode n;
-        out_node_rec(F, uniform_load(F.nrec + (u - F.num_leaves)), n);
+        const NodeRec r = uniform_load(F.nrec + (u - F.num_leaves));
+        out_node_rec(F, r, n);
         double acc[2][5];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int j = 0; j < 5; ++j) acc[a][j] = 0.0;
         double rt = 0.0;
-        if (k < K) outside_site<kNoCherry, kNoCherry>(F, n, k, acc, rt);
+        if constexpr (W == 64) {
+          if (k < K) outside_site_q(F, n, k, acc, rt, chain_q_row(F, r.ch0, r.ro0), chain_q_row(F, r.ch1, r.ro1));
+        } else {
+          if (k < K) outside_site<kNoCherry, kNoCherry>(F, n, k, acc, rt);
+        }
         out_write_partials<W>(F, u, n.ch, lane, chunk, chunk + 1, acc, rt);
       }
       __syncthreads();
