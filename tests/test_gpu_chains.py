@@ -404,3 +404,43 @@ def test_outside_packs_equal_one_chain_per_wavefront(cap, monkeypatch):
     for other in res[1:]:
         for a, b in zip(res[0], other):
             np.testing.assert_array_equal(a, b)
+
+
+def test_chain_prior_weight_rows_with_unnormalised_priors(monkeypatch):
+    """Per-site prior masses other than 1 (the fastEM model's observed-state
+    alphabets): the prior-weight rows hold the mass for labels that name no
+    edit state, and chain passes (packed and not) still equal level passes
+    bit for bit, E-step and EM."""
+    from laml_amd.forest import SiteTable
+
+    rng = np.random.Generator(np.random.PCG64(2301))
+    tree, Q, codes, _ = sim.make_problem(1500, 150, 4, 25, 1.0, 0.1, 0.2, 2302)
+    scale = rng.uniform(0.3, 0.9, size=len(Q))
+    Qs = [{s: p * scale[k] for s, p in Q[k].items()} for k in range(len(Q))]
+    sites = SiteTable(Qs, normalized=False)
+    assert np.abs(sites.qsum - 1.0).min() > 0.05
+    spec = sim.problem_spec(tree, Qs, codes, sites=sites)
+    dev = torch.cuda.current_device()
+    monkeypatch.setenv("LAML_PACK_WAVES", "1")
+    res = []
+    for on, pack in ((1, 8), (1, 1), (False, 1)):
+        monkeypatch.setenv("LAML_CHAIN_PACK", str(pack))
+        print("unnormalised priors, chains", on, "pack", pack, flush=True)
+        F = Forest([spec], device=dev)
+        F.set_pass_mode(2)
+        H = F.num_levels
+        F.set_shapes(np.ones(H, dtype=np.int32), np.ones(H, dtype=np.int32), min(9, H))
+        F.set_chains(on, 0, 0)
+        assert F.chains() == (1 if on else 0)
+        d = torch.tensor(F.lengths_to_level([tree.length]), dtype=torch.float64, device=F.device)
+        nu = torch.full((1,), 0.1, dtype=torch.float64, device=F.device)
+        phi = torch.full((1,), 0.2, dtype=torch.float64, device=F.device)
+        counts, rt, nll = F.estep(d, nu, phi)
+        e = (counts.cpu().numpy().copy(), rt.cpu().numpy().copy(), nll.cpu().numpy().copy())
+        nll2, _ = F.em(d, nu, phi, 6, 0.0)
+        res.append(e + (nll2.cpu().numpy().copy(), d.cpu().numpy().copy(), nu.cpu().numpy().copy(),
+                        phi.cpu().numpy().copy()))
+        F.close()
+    for other in res[1:]:
+        for a, b in zip(res[0], other):
+            np.testing.assert_array_equal(a, b)
