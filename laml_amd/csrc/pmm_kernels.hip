@@ -1296,13 +1296,13 @@ typedef const __attribute__((address_space(1))) void glb_void;
 // lane's 16-bit code), and in a pack the next head's outside state.
 constexpr int kChainLdsBeta = 0;       // [2][64] double2
 constexpr int kChainLdsBE = 2048;      // [2][64] int32
-constexpr int kChainLdsQ = 2560;       // [2][64] double (1 KiB of the 2 KiB region used)
-constexpr int kChainLdsCoef = 4608;    // [64] uint32 (20 used)
-constexpr int kChainLdsCode = 4864;    // [2 children][64] uint32 (512 of 1024 bytes used)
-constexpr int kChainLdsHeadA = 5888;   // [2 halves][64] 16 bytes: a head's outside quad
-constexpr int kChainLdsHeadE = 7936;   // [64] int32: its exponent
-constexpr int kChainLdsHeadG = 8192;   // [64] uint32: the word holding its specific state
-constexpr int kChainLdsBytes = 8448;
+constexpr int kChainLdsQ = 2560;       // [2][64] double
+constexpr int kChainLdsCoef = 3584;    // [64] uint32 (20 used)
+constexpr int kChainLdsCode = 3840;    // [2][64] uint32
+constexpr int kChainLdsHeadA = 4352;   // [2 halves][64] 16 bytes: a head's outside quad
+constexpr int kChainLdsHeadE = 6400;   // [64] int32: its exponent
+constexpr int kChainLdsHeadG = 6656;   // [64] uint32: the word holding its specific state
+constexpr int kChainLdsBytes = 6912;
 
 // copy `size` (4 or 16) bytes at src (per lane) to dst + size * lane; rows
 // read once per pass carry the non-temporal hint, so that the per-site
@@ -1353,13 +1353,12 @@ __device__ __forceinline__ int code_of_word(uint32_t wd, unsigned kk) {
   return (int)(int16_t)(kk & 1u ? wd >> 16 : wd & 0xffffu);
 }
 
-// Copy step r's codes into code slot `slot` (0).
-__device__ __forceinline__ void chain_stage_codes(const DevForest& F, const ChainStep& r, unsigned kk, int slot,
-                                                  char* lds) {
+// Copy step r's codes.
+__device__ __forceinline__ void chain_stage_codes(const DevForest& F, const ChainStep& r, unsigned kk, char* lds) {
   const int16_t* row0 = chain_code_row(F, r.ch0, r.ro0) + (kk & ~1u);
   const int16_t* row1 = chain_code_row(F, r.ch1, r.ro1) + (kk & ~1u);
-  to_lds(row0, lds + kChainLdsCode + 512 * slot, 4, true);
-  to_lds(row1, lds + kChainLdsCode + 512 * slot + 256, 4, true);
+  to_lds(row0, lds + kChainLdsCode, 4, true);
+  to_lds(row1, lds + kChainLdsCode + 256, 4, true);
 }
 
 // Copy step r's operands: branch coefficients of both children, inside rows
@@ -1502,7 +1501,7 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
 
   // prologue: step 0's operands and codes and step 1's codes staged
   {
-    chain_stage_codes(F, r0, kk, 0, lds);
+    chain_stage_codes(F, r0, kk, lds);
     chain_stage(F, r0, k0, kk, lane, lds);
   }
   int touch = 0;  // first word of the record four steps ahead (brings its line into the scalar cache)
@@ -1549,7 +1548,7 @@ __global__ void __launch_bounds__(256) k_outside_chain(DevForest F, const int2* 
     auto stage = [&]() {
       if (more) {
         chain_stage(F, r1, k0, kk, lane, lds);
-        chain_stage_codes(F, r1, kk, 0, lds);
+        chain_stage_codes(F, r1, kk, lds);
         if (head1) chain_stage_head(F, ru1, kk, lds);
       }
     };
