@@ -43,9 +43,18 @@ def segment_width(max_k):
     return 8 if max_k <= 8 else 16 if max_k <= 16 else 32 if max_k <= 32 else 64
 
 
+# Device memory a forest takes beyond estimate_forest_bytes, whatever its
+# size: allocation rounding and the caching allocator's segments of the
+# forest's small tensors (measured at 2-37 MB); the server reserves it once
+# per slot.
+FOREST_FIXED_BYTES = 48 << 20
+
+
 def estimate_forest_bytes(specs):
-    """Device bytes of a forest of these problems (rows padded to the segment
-    width; per node metadata, partials and parameters; per site tables)."""
+    """Device bytes of a forest of these problems that grow with it (rows
+    padded to the segment width; per node metadata, records, chain plans,
+    partials and parameters; per site tables); with FOREST_FIXED_BYTES, an
+    upper bound on what its creation and an EM call take."""
     K = [p.sites.K for p in specs]
     W = segment_width(max(K))
     chunks = -(-max(K) // W)
@@ -62,7 +71,10 @@ def estimate_forest_bytes(specs):
             total += n_leaf * kp * (2 + q) + n_int * kp * (60 + q)
             nodes += t.n_nodes
         total += p.sites.K * (8 + 4 + 8 + 4 + 4) + 8 * int(p.sites.qoff[-1])
-    total += nodes * (4 * 8 + 8 * 3 + 32 + 48 * chunks + 8 * 8)
+    # per node: coefficients 40, counts 40, nu solve 24, node record 64,
+    # ids and offsets 48, chain plan 48; per 64-site chunk: partial counts 40
+    # and dropout sum 8
+    total += nodes * (264 + 48 * chunks)
     return int(total) + (1 << 20)
 
 
@@ -173,7 +185,7 @@ class PMMServer:
         self.batches = []  # sizes of the batches solved
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(device)
-            hbm_budget = int(free * 0.8)
+            hbm_budget = int(free * 0.8) - self.slots * FOREST_FIXED_BYTES
         self.budget = int(hbm_budget)
         self.in_use = 0
         self.peak_in_use = 0

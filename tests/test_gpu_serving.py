@@ -102,3 +102,37 @@ def test_server_batches_small_requests():
     assert max(srv.batches) > 1 and sum(srv.batches) == len(reqs)
     for r in res:
         _close((r.nll, r.lengths, r.nu, r.phi, r.iters), alone[r.rid])
+
+
+def test_footprint_estimate_bounds_device_use():
+    """The admission estimate of a request plus the per-forest allowance the
+    server reserves per slot is at least the device memory its forest takes
+    through creation and an EM call (free memory before and after, with the
+    caching allocator emptied first; a first forest loads the code objects
+    and static arrays beforehand)."""
+    from laml_amd.forest import Forest
+    from laml_amd.serving import FOREST_FIXED_BYTES, estimate_forest_bytes
+
+    dev = torch.cuda.current_device()
+
+    def run(n, K):
+        tree, Q, codes, _ = sim.make_problem(n, K, 5, 50, 1.0, 0.1, 0.1, 9)
+        spec = sim.problem_spec(tree, Q, codes)
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        f0, _ = torch.cuda.mem_get_info(dev)
+        F = Forest([spec], device=dev)
+        d = torch.tensor(F.lengths_to_level([np.full(tree.n_nodes, 0.05)]), dtype=torch.float64, device=F.device)
+        nu = torch.tensor([0.5], dtype=torch.float64, device=F.device)
+        phi = nu.clone()
+        F.em(d, nu, phi, 5, 1e-8)
+        torch.cuda.synchronize()
+        f1, _ = torch.cuda.mem_get_info(dev)
+        F.close()
+        return estimate_forest_bytes([spec]) + FOREST_FIXED_BYTES, f0 - f1
+
+    run(300, 8)
+    for n, K in [(2000, 20), (2000, 40), (3000, 65), (5000, 300), (20000, 300)]:
+        est, used = run(n, K)
+        print("footprint", n, K, est, used, flush=True)
+        assert est >= used, (n, K, est, used)
