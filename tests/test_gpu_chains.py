@@ -444,3 +444,31 @@ def test_chain_prior_weight_rows_with_unnormalised_priors(monkeypatch):
     for other in res[1:]:
         for a, b in zip(res[0], other):
             np.testing.assert_array_equal(a, b)
+
+
+def test_benchmark_batch_chain_passes_equal_levels():
+    """The benchmark's batch (16 problems, each a 10,000-leaf Yule tree x 300
+    sites) with the default pack settings, so that the big chain rounds run
+    packed and the small ones a chain per wavefront: the E-step equals level
+    passes with one chunk per wavefront bit for bit."""
+    probs = [sim.make_problem(10000, 300, 5, 60, 1.0, 0.1, 0.1, 20261206 + 1000 * b) for b in range(16)]
+    specs = [sim.problem_spec(t, q, c) for (t, q, c, _) in probs]
+    lengths = [np.full(t.n_nodes, 0.05) for (t, _, _, _) in probs]
+    dev = torch.cuda.current_device()
+    res = []
+    for on in (1, False):
+        print("benchmark batch, chains", on, flush=True)
+        F = Forest(specs, device=dev)
+        F.set_pass_mode(2)
+        H = F.num_levels
+        F.set_shapes(np.ones(H, dtype=np.int32), np.ones(H, dtype=np.int32), 18)
+        F.set_chains(on, 0, 0)
+        assert F.chains() == (1 if on else 0)
+        d = torch.tensor(F.lengths_to_level(lengths), dtype=torch.float64, device=F.device)
+        nu = torch.full((16,), 0.5, dtype=torch.float64, device=F.device)
+        phi = torch.full((16,), 0.5, dtype=torch.float64, device=F.device)
+        counts, rt, nll = F.estep(d, nu, phi)
+        res.append((counts.cpu().numpy().copy(), rt.cpu().numpy().copy(), nll.cpu().numpy().copy()))
+        F.close()
+    for a, b in zip(*res):
+        np.testing.assert_array_equal(a, b)
